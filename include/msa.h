@@ -1,0 +1,139 @@
+/*
+ * msa.h -- C ABI of libmsa_hip.so, the device layer of the micSA path on
+ * MI355X (gfx950).  Plain pointers and sizes, no HIP or C++ types.
+ *
+ * This library replaces the CUDA-aware host functions of the reference
+ * (PROGRAM/src/main.cu, prototypes in PROGRAM/include/header.h:329-340).
+ * Each entry point names the reference function(s) it stands in for.
+ *
+ * Data model (sample-per-lane): every uint64 word holds one site of 64
+ * independent samples, bit b = sample b.  Host-side arrays use the canonical
+ * site order s = x + L*(y + L*z):
+ *   J     [3][V]        bond from s to s+e_d, d = x,y,z (one bit per sample)
+ *   spin  [R][V]        R replicas share J
+ *   d0,d1 [R][nc][V]    the two bit planes of the 2-bit demons (D = d0 + 2 d1)
+ * On the device the same words are stored colour-split.
+ *
+ * Every call returns MSA_OK (0) or a negative msa_status; msa_strerror()
+ * describes the last error of the calling thread.  A context is not
+ * thread-safe: one host thread drives it, as one thread drives the
+ * reference's device (main.cu:137-180).
+ */
+#ifndef MSA_H
+#define MSA_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum msa_status {
+  MSA_OK = 0,
+  MSA_ERR_ARG = -1,     /* invalid argument / geometry */
+  MSA_ERR_HIP = -2,     /* HIP runtime error */
+  MSA_ERR_NOMEM = -3,   /* device allocation failed */
+  MSA_ERR_STATE = -4,   /* call out of order (e.g. run before tables are set) */
+  MSA_ERR_RANGE = -5    /* refresh or LUT index outside the uploaded tables */
+};
+
+typedef struct msa_ctx msa_ctx;
+
+typedef struct {
+  int32_t L;    /* linear size, even, 4..1024 (reference: compile-time L, header.h:38-51) */
+  int32_t R;    /* replicas sharing J, 1..16 (reference: NR = 16, header.h:17) */
+  int32_t nc;   /* demon layers, 1..6 (reference: data.nc, input.in:6, main.cu:1082) */
+} msa_geom;
+
+typedef struct { uint32_t x, y, z, w; } msa_u32x4;   /* Philox counter (CUDA uint4) */
+
+/* Heat-bath table for eight iid Bernoulli(q) bits per byte, byte-identical to
+ * s_lut_heat_bath with NUMBITSPREBUSQUEDAS = 10 (header.h:113-116):
+ * 256 x (msb32, lsb32) cumulative thresholds, then 1024 x uint16 presearch
+ * entries (min | max << 8).  This is the payload of the reference LUT file
+ * after its leading double (io.c:561-591). */
+typedef struct {
+  uint32_t umbrales[512];
+  uint32_t prebusqueda[512];
+} msa_lut;
+
+/* Allocate device state for one lattice on `device` and create the
+ * context's stream.  Replaces CopyConstOnDevice (main.cu:379-402) and the
+ * allocations of CopyDataOnDevice / mmcuda (main.cu:404-471, 1209-1289). */
+int msa_create(int device, const msa_geom *geom, msa_ctx **out);
+
+/* Release everything.  Replaces FreeDevice (main.cu:508-530). */
+int msa_destroy(msa_ctx *ctx);
+
+/* Upload couplings, spins and demons (canonical order, see above); d0/d1 may
+ * be NULL for all-zero demons.  Replaces the uploads of CopyDataOnDevice
+ * (main.cu:404-471) and of refresh_walkers (main.cu:355-362). */
+int msa_upload_state(msa_ctx *ctx, const uint64_t *J, const uint64_t *spin,
+                     const uint64_t *d0, const uint64_t *d1);
+
+/* Download spins and demons (any pointer may be NULL) and couplings.
+ * Replaces copy_from_GPU_for_write (main.cu:473-490). */
+int msa_download_state(msa_ctx *ctx, uint64_t *spin, uint64_t *d0, uint64_t *d1);
+int msa_download_couplings(msa_ctx *ctx, uint64_t *J);
+
+/* Counter-based construction on the device: J word (d,s) and spin word (r,s)
+ * are Philox4x32-10 outputs of counter {s, d|r, 1|2, 0} and key
+ * {seed_lo, seed_hi}; demons are cleared.  No reference counterpart: the
+ * reference constructs state on the host (ini.c:275-299, 425-519); this is
+ * the construction used by the batched and served forms. */
+int msa_init_philox(msa_ctx *ctx, uint64_t seed);
+
+/* Upload `nlut` pairs of heat-bath tables (luts[2*i] for demon bit plane 0,
+ * luts[2*i+1] for plane 1).  Replaces the LUT upload of CopyConstOnDevice
+ * (main.cu:396-398), generalised to one pair per temperature step. */
+int msa_set_luts(msa_ctx *ctx, int32_t nlut, const msa_lut *luts);
+
+/* Upload the refresh schedule: for refresh k (0 <= k < nref) and replica r,
+ * the Philox counter ctr[k*R + r] (built like _fill_bits, fill_bits.h:47-68),
+ * the per-replica keys key[r] (seed_keys, random_generator.cu:150-156) and
+ * the LUT pair index lut_idx[k].  No single reference counterpart: the
+ * reference refreshes demons on host threads (refresh_walkers main.cu:340-377,
+ * InitWalkers ini.c:712-781); here the refresh runs on the device, fused
+ * into the first sweep that uses each demon layer. */
+int msa_set_refresh(msa_ctx *ctx, int32_t nref, const msa_u32x4 *ctr, const uint32_t *key,
+                    const int32_t *lut_idx);
+
+/* Run iterations t0 .. t0+niter-1.  One iteration = optional demon refresh
+ * of every layer, nc sweeps (colour 0 then colour 1, sweep ic using layer ic)
+ * and the demon walk (main.cu:137-150, Monte_Carlo main.cu:254-306 with
+ * flag = 1).  Refresh cadence:
+ *   refresh_every  > 0 : refresh when t % refresh_every == 0, k = t / refresh_every
+ *   refresh_every == 0 : never
+ *   refresh_every == -1: at t = 0 and powers of two (main.cu:140-142),
+ *                        k = 0 for t = 0, k = log2(t) + 1 otherwise
+ * Asynchronous on the context stream.  Replaces the Monte_Carlo launches. */
+int msa_run(msa_ctx *ctx, uint64_t t0, int64_t niter, int64_t refresh_every);
+
+/* Same as msa_run, then wait and report the elapsed device time of the
+ * launches (HIP events on the context stream) in *ms. */
+int msa_run_timed(msa_ctx *ctx, uint64_t t0, int64_t niter, int64_t refresh_every, float *ms);
+
+/* Integer observables per sample lane (each array [..][64], NULL to skip):
+ *   unsat[R][64]   unsatisfied bonds; energy h_Ener = 3V - 2*unsat
+ *                  (d_computeEne main.cu:745-815, Measure_Energy_Spins main.cu:315-336)
+ *   demon[R][64]   sum over layers and sites of D0 + 2 D1
+ *                  (Measure_Energy_Clerics med.c:55-83, on the device here)
+ *   q[R/2][64]     sites where replicas 2p and 2p+1 differ
+ *   qlink[R/2][64] bonds where s_x s_y differs between replicas 2p and 2p+1
+ * q and qlink have no reference counterpart.  Blocks until done. */
+int msa_measure(msa_ctx *ctx, int32_t *unsat, int32_t *demon, int32_t *q, int32_t *qlink);
+
+/* Wait for all work on the context stream. */
+int msa_sync(msa_ctx *ctx);
+
+/* Bytes of device memory held by a context of this geometry (for admission
+ * by HBM footprint). */
+int64_t msa_footprint_bytes(const msa_geom *geom);
+
+/* Description of the last error on the calling thread (never NULL). */
+const char *msa_strerror(int status);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
