@@ -23,6 +23,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 HBM_PEAK_GBS = 8000.0
 
@@ -72,6 +74,108 @@ def cpu_baseline(L, R, nc, budget_s, threads):
                       "%.1f s" % (n, L, R, dt)}
 
 
+def splitmix64(x):
+    x = (x + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+    z = x
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+    return x, z ^ (z >> 31)
+
+
+class Xoshiro256pp:
+    """xoshiro256++ (Vigna), state seeded from splitmix64(seed)."""
+
+    def __init__(self, seed):
+        self.s = []
+        x = seed
+        for _ in range(4):
+            x, v = splitmix64(x)
+            self.s.append(v)
+
+    def next(self):
+        M = 0xFFFFFFFFFFFFFFFF
+        s = self.s
+        rotl = lambda v, k: ((v << k) | (v >> (64 - k))) & M
+        out = (rotl((s[0] + s[3]) & M, 23) + s[0]) & M
+        t = (s[1] << 17) & M
+        s[2] ^= s[0]; s[3] ^= s[1]; s[1] ^= s[2]; s[0] ^= s[3]; s[2] ^= t
+        s[3] = rotl(s[3], 45)
+        return out
+
+
+def batched_vs_plain(pkg, n_lat, len_div, block=256, chunk=64):
+    """The batched configuration scaled down: n_lat lattices, L uniform over
+    {16,24,32,48,64,96}, lengths uniform over {4096,16384,65536}/len_div,
+    T 2.0->0.7, refresh every sweep, R=1; drawn with xoshiro256++ seeded 4242.
+    Plain = each lattice alone through msa_run (two launches per sweep);
+    batched = all lattices in one msa_batch (graph-captured launches)."""
+    import importlib
+    serve = importlib.import_module("micsa_ea_mc_amd.serve")
+    msa = importlib.import_module("micsa_ea_mc_amd.msa")
+    rng = Xoshiro256pp(4242)
+    sizes = (16, 24, 32, 48, 64, 96)
+    lens = (4096 // len_div, 16384 // len_div, 65536 // len_div)
+    lats = [(sizes[rng.next() % 6], lens[rng.next() % 3]) for _ in range(n_lat)]
+    tables = serve.Tables()
+    jobs = []
+    for i, (L, n) in enumerate(lats):
+        req = serve.Request(i, L, n, 2.0, 0.7, 77 + i, 0.0)
+        jobs.append((req, tables.job(req, i)))
+    updates = sum(64 * L ** 3 * n for L, n in lats)
+    # plain
+    t_plain = 0.0
+    for req, job in jobs:
+        lat = pkg.Lattice(req.L, 1, 1)
+        lat.init_philox(req.seed)
+        lat.set_luts(tables.luts)
+        ctr = np.ctypeslib.as_array(job.ctr, shape=(job.nref * 4,)).copy()
+        idx = np.ctypeslib.as_array(job.lut_idx, shape=(job.nref,)).copy()
+        key = np.ctypeslib.as_array(job.key, shape=(1,)).copy()
+        lat.set_refresh(ctr, key, idx)
+        lat.sync()
+        t0 = time.perf_counter()
+        lat.run(0, req.niter, 1)
+        lat.sync()
+        t_plain += time.perf_counter() - t0
+        lat.close()
+    # batched
+    b = msa.Batch(tables.luts)
+    b.set_shape(block, chunk)
+    for req, job in jobs:
+        b.add(job)
+    b.sync()
+    b.run(0)
+    b.sync()
+    t0 = time.perf_counter()
+    b.run(max(n for _, n in lats))
+    b.sync()
+    t_batch = time.perf_counter() - t0
+    t, te = b.progress(len(lats))
+    assert (t == te).all()
+    b.close()
+    return {"lattices": n_lat, "length_divisor": len_div, "block": block, "graph_chunk": chunk,
+            "spin_updates": updates,
+            "plain_spin_updates_per_ps": round(updates / t_plain / 1e12, 4),
+            "batched_spin_updates_per_ps": round(updates / t_batch / 1e12, 4),
+            "batched_over_plain": round(t_plain / t_batch, 3),
+            "plain_s": round(t_plain, 3), "batched_s": round(t_batch, 3)}
+
+
+def served(pkg, n_req, rate, log2_len, max_bytes, epoch=64):
+    """Served configuration scaled down: Poisson arrivals (seed 5151), L uniform
+    over {16,32,48,64,96,128}, length 2^U{log2_len}, T 2.0 -> U[0.6,0.9],
+    refresh every sweep, admission by device footprint."""
+    import importlib
+    serve = importlib.import_module("micsa_ea_mc_amd.serve")
+    reqs = serve.make_requests(n_req, rate, seed=5151, log2_len=log2_len)
+    _, summ = serve.serve(reqs, max_bytes=max_bytes, epoch=epoch)
+    summ.update({"arrival_rate_per_s": rate, "log2_length": list(log2_len),
+                 "max_bytes": max_bytes})
+    for k in ("makespan_s", "spin_updates_per_ps", "latency_p50_s", "latency_p99_s"):
+        summ[k] = round(summ[k], 4)
+    return summ
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,6 +188,11 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batched", type=int, default=48, help="lattices in the batched test (0: skip)")
+    ap.add_argument("--batched-len-div", type=int, default=64)
+    ap.add_argument("--served", type=int, default=60, help="requests in the served test (0: skip)")
+    ap.add_argument("--served-rate", type=float, default=40.0)
+    ap.add_argument("--autotune", action="store_true", help="autotune the sweep kernel first")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the lattice is not sharded); "
@@ -95,6 +204,10 @@ def main():
     nsched = 1 << 18
     lat = pkg.Lattice(L, R, nc, device=0)
     lat.init_philox(0x5EED)
+    tuned = None
+    if args.autotune:
+        tuned = lat.autotune(8)
+    tiling = lat.tiling()
     total = args.warmup + args.steps
     sched = pkg.schedule(1.8, 0.6, nsched, every)
     ctr, keys, luts, idx = pkg.schedule_tables(sched, total, R, [3003 + r for r in range(R)], 0)
@@ -137,6 +250,8 @@ def main():
                         "refresh every %d sweeps" % (L, R, nc, every),
             "L": L, "R": R, "nc": nc, "lanes": 64, "refresh_every": every,
             "sweeps_timed": args.steps, "refreshes_timed": nref,
+            "kernel_mode": "one-pass" if tiling[0] else "two-pass",
+            "tile_rows_planes": [tiling[1], tiling[2]],
         },
         "roofline": {
             "bound": "hbm",
@@ -152,6 +267,12 @@ def main():
         "wall_s": round(wall, 4),
         "mean_energy_per_site": round(float(-m["energy"].mean()) / V, 6),
     }
+    if tuned is not None:
+        out["autotune_ms_per_sweep"] = round(tuned[3], 5)
+    if args.batched > 0:
+        out["batched"] = batched_vs_plain(pkg, args.batched, args.batched_len_div)
+    if args.served > 0:
+        out["served"] = served(pkg, args.served, args.served_rate, (6, 11), 8 << 30)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(L, R, nc, args.cpu_budget, args.cpu_threads)
     print(json.dumps(out))

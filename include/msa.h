@@ -123,6 +123,69 @@ int msa_run_timed(msa_ctx *ctx, uint64_t t0, int64_t niter, int64_t refresh_ever
  * q and qlink have no reference counterpart.  Blocks until done. */
 int msa_measure(msa_ctx *ctx, int32_t *unsat, int32_t *demon, int32_t *q, int32_t *qlink);
 
+/* Kernel choice for sweeps without a refresh (no reference counterpart; the
+ * autotuner and the batched form set it).  mode 0: two kernels per sweep,
+ * one per colour.  mode 1: one kernel per sweep whose blocks own ty rows x
+ * tz planes (ty, tz dividing L; 0,0 = built-in default).  Sweeps with a
+ * refresh always use the two-kernel form.  Results are identical in every
+ * mode; only the speed differs. */
+int msa_set_tiling(msa_ctx *ctx, int32_t mode, int32_t ty, int32_t tz);
+int msa_get_tiling(msa_ctx *ctx, int32_t *mode, int32_t *ty, int32_t *tz);
+
+/* Autotuner (no reference counterpart): times `sweeps` sweeps of every
+ * kernel choice valid for this geometry on a scratch lattice of the same
+ * geometry (the context's own state is not touched), applies the fastest and
+ * remembers it for the process; later calls for the same (L, R, nc) reuse it.
+ * Outputs may be NULL. */
+int msa_autotune(msa_ctx *ctx, int32_t sweeps, int32_t *mode, int32_t *ty, int32_t *tz,
+                 float *ms_per_sweep);
+
+/* ------------------------------------------------------------ batches --
+ * The batched form (no reference counterpart; the reference farms samples
+ * out to separate processes, main.cu:33-50): many lattices of uneven L, each
+ * with its own length and refresh schedule, advance together -- every launch
+ * covers all unfinished lattices and a captured HIP graph replays `chunk`
+ * iterations at a time.  Each lattice's results are bit-identical to msa_run
+ * on the same lattice with the same tables.  Batched lattices have nc = 1. */
+typedef struct msa_batch msa_batch;
+
+typedef struct {
+  msa_geom geom;            /* nc must be 1 */
+  uint64_t seed;            /* state built as msa_init_philox(seed) */
+  int64_t niter;            /* iterations this lattice runs */
+  int64_t refresh_every;    /* cadence as in msa_run */
+  int32_t nref;             /* refresh tables as in msa_set_refresh ... */
+  const msa_u32x4 *ctr;     /* [nref][R] */
+  const uint32_t *key;      /* [R] */
+  const int32_t *lut_idx;   /* [nref], into the batch's shared LUT table */
+} msa_job;
+
+/* max_bytes bounds the device memory of the admitted lattices (<= 0: none). */
+int msa_batch_create(int device, int64_t max_bytes, msa_batch **out);
+int msa_batch_destroy(msa_batch *b);
+/* Shared LUT pairs, before the first msa_batch_add. */
+int msa_batch_set_luts(msa_batch *b, int32_t nlut, const msa_lut *luts);
+/* Admit a lattice; MSA_ERR_NOMEM when its footprint (msa_batch_job_bytes)
+ * does not fit under max_bytes next to those admitted. */
+int msa_batch_add(msa_batch *b, const msa_job *job, int32_t *slot);
+int64_t msa_batch_job_bytes(const msa_job *job);
+int64_t msa_batch_bytes_in_use(msa_batch *b);
+/* Replace the constructed state of a lattice that has not run yet. */
+int msa_batch_upload_state(msa_batch *b, int32_t slot, const uint64_t *J, const uint64_t *spin,
+                           const uint64_t *d0, const uint64_t *d1);
+/* Launch shape: threads per block (128, 256, 512) and iterations per graph. */
+int msa_batch_set_shape(msa_batch *b, int32_t block, int32_t chunk);
+/* Advance every unfinished lattice by up to niter iterations (asynchronous). */
+int msa_batch_run(msa_batch *b, int64_t niter);
+int msa_batch_sync(msa_batch *b);
+/* Iteration reached and length of slots 0..nslots-1 (-1 for empty slots). */
+int msa_batch_progress(msa_batch *b, int32_t nslots, int64_t *t, int64_t *tend);
+int msa_batch_measure(msa_batch *b, int32_t slot, int32_t *unsat, int32_t *demon, int32_t *q,
+                      int32_t *qlink);
+int msa_batch_download_state(msa_batch *b, int32_t slot, uint64_t *spin, uint64_t *d0, uint64_t *d1);
+/* Retire a lattice and free its memory. */
+int msa_batch_remove(msa_batch *b, int32_t slot);
+
 /* Wait for all work on the context stream. */
 int msa_sync(msa_ctx *ctx);
 

@@ -102,6 +102,15 @@ int msa_sched_tables(const msa_sched *s, int64_t nref, int R, const uint64_t *se
                      uint32_t jobid, msa_u32x4 *ctr, uint32_t *keys, msa_lut *luts,
                      int32_t *lut_idx);
 
+/* Shared-table form for batches and serving: temperatures are rounded to the
+ * grid T = tgrid * n, n = nlo .. nlo+nlut-1, and every lattice indexes one
+ * table of LUT pairs built once (msa_lut_grid).  The refresh at iteration t
+ * uses n = round(T(t) / tgrid) clamped to the grid. */
+void msa_lut_grid(double tgrid, int32_t nlo, int32_t nlut, msa_lut *luts);
+int  msa_sched_tables_grid(const msa_sched *s, int64_t nref, int R, const uint64_t *seed_MC,
+                           uint32_t jobid, double tgrid, int32_t nlo, int32_t nlut,
+                           msa_u32x4 *ctr, uint32_t *keys, int32_t *lut_idx);
+
 /* ------------------------------------------------------ file formats -- */
 
 #define MSA_MAXSTR 1024

@@ -1,0 +1,471 @@
+// msa_batch.hip -- the batched form: many lattices of uneven L advanced in
+// the same kernel launches (see msa_batch_* in include/msa.h).
+//
+// Every lattice is a full msa_ctx (its own ping-pong state and refresh
+// tables); the batch adds a device array of lattice descriptors, a work list
+// of (lattice, tile) items covering every active lattice, and a captured HIP
+// graph of `chunk` iterations (two launches each: colour 0, colour 1).  Each
+// lattice keeps its iteration counter t on the device; the last block of a
+// lattice to finish colour 1 advances it, so the captured graph can be
+// replayed any number of times and lattices that reach their length stop by
+// themselves.  Integer results are those of msa_run on the same lattice.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "msa_common.hpp"
+#include "msa_ctx.hpp"
+
+struct BLat {
+  uint64_t *sp[2];
+  const uint64_t *jp;
+  ulonglong2 *dm[2];
+  msa::Geo g;
+  const uint4 *ctr;
+  const uint32_t *key;
+  const msa_lut *luts;
+  const int32_t *lut_idx;
+  int64_t every;
+  uint64_t t, tend;
+  uint32_t ntiles, done;
+  int32_t nref, pad;
+};
+
+struct BItem {
+  uint32_t lat, tile;
+};
+
+// One colour of one sweep for every (lattice, tile) item: the update of
+// k_halfsweep (d_oneMCstepBN_multisample + fused walk + fused refresh) for
+// nc = 1, with the lattice's own geometry, buffers, counters and tables.
+template <int BS>
+__global__ void __launch_bounds__(BS) k_batch_halfsweep(BLat *lats, const BItem *items, int colour) {
+  __shared__ uint32_t s_lut[2 * 1024];
+  const BItem it = items[blockIdx.x];
+  BLat *lp = lats + it.lat;
+  const uint64_t t = lp->t;
+  if (t >= lp->tend) return;                     // whole block: lattice finished
+  const msa::Geo g = lp->g;
+  const int64_t k = refresh_index(t, lp->every);
+  const bool refresh = k >= 0 && k < lp->nref;
+  if (refresh) {
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(lp->luts + 2 * (size_t)lp->lut_idx[k]);
+    for (uint32_t j = threadIdx.x; j < 2048; j += BS) s_lut[j] = src[j];
+    __syncthreads();
+  }
+  const int par = (int)(t & 1u);
+  const uint64_t *A = lp->sp[par];
+  uint64_t *B = lp->sp[par ^ 1];
+  const ulonglong2 *dmi = lp->dm[par];
+  ulonglong2 *dmo = lp->dm[par ^ 1];
+  const uint64_t h = (uint64_t)it.tile * BS + threadIdx.x;
+  if (h < g.H) {
+    const int col = colour;
+    const uint32_t i = (uint32_t)(h % g.Lh);
+    const uint64_t yz = h / g.Lh;
+    const uint32_t y = (uint32_t)(yz % g.L), z = (uint32_t)(yz / g.L);
+    const uint32_t off = (y + z + (uint32_t)col) & 1u;
+    const uint32_t x = 2u * i + off;
+    const uint32_t ip = off ? msa::wrapi((int)i + 1, g.Lh) : i;
+    const uint32_t im = off ? i : msa::wrapi((int)i - 1, g.Lh);
+    const uint32_t yp = msa::wrapi((int)y + 1, g.L), ym = msa::wrapi((int)y - 1, g.L);
+    const uint32_t zp = msa::wrapi((int)z + 1, g.L), zm = msa::wrapi((int)z - 1, g.L);
+    const uint64_t rb = (uint64_t)g.Lh * (y + (uint64_t)g.L * z);
+    const uint64_t hxp = ip + rb, hxm = im + rb;
+    const uint64_t hyp = i + (uint64_t)g.Lh * (yp + (uint64_t)g.L * z);
+    const uint64_t hym = i + (uint64_t)g.Lh * (ym + (uint64_t)g.L * z);
+    const uint64_t hzp = i + (uint64_t)g.Lh * (y + (uint64_t)g.L * zp);
+    const uint64_t hzm = i + (uint64_t)g.Lh * (y + (uint64_t)g.L * zm);
+    const uint64_t *Jc = lp->jp + (uint64_t)col * 3 * g.H;
+    const uint64_t *Jo = lp->jp + (uint64_t)(col ^ 1) * 3 * g.H;
+    const uint64_t J0 = Jc[h], J1 = Jc[g.H + h], J2 = Jc[2 * g.H + h];
+    const uint64_t J3 = Jo[hxm], J4 = Jo[g.H + hym], J5 = Jo[2 * g.H + hzm];
+    const uint64_t hw = msa::walk_dest(g, col, 1, 0, 0, i, y, z);   // layer 0 walks along +x
+    const uint64_t *Sn = col ? B : A;
+    msa::LutView t0{reinterpret_cast<const uint2 *>(s_lut),
+                    reinterpret_cast<const uint16_t *>(s_lut + 512)};
+    msa::LutView t1{reinterpret_cast<const uint2 *>(s_lut + 1024),
+                    reinterpret_cast<const uint16_t *>(s_lut + 1536)};
+    const uint32_t tid = (uint32_t)(x + (uint64_t)g.L * (y + (uint64_t)g.L * z));
+    for (uint32_t r = 0; r < g.R; ++r) {
+      const uint64_t *So = Sn + (uint64_t)(2 * r + (col ^ 1)) * g.H;
+      const uint64_t sown = A[(uint64_t)(2 * r + col) * g.H + h];
+      const uint64_t dbase = ((uint64_t)r * 2 + col) * g.H;
+      uint64_t D0, D1;
+      if (refresh) {
+        msa::refresh_demon(t0, t1, lp->ctr[(size_t)k * g.R + r], lp->key[r], tid, D0, D1);
+      } else {
+        const ulonglong2 d = dmi[dbase + h];
+        D0 = d.x;
+        D1 = d.y;
+      }
+      const uint64_t sn = msa::site_update(sown, So[hxp], So[hyp], So[hzp], So[hxm], So[hym],
+                                           So[hzm], J0, J1, J2, J3, J4, J5, D0, D1);
+      B[(uint64_t)(2 * r + col) * g.H + h] = sn;
+      dmo[dbase + hw] = make_ulonglong2(D0, D1);
+    }
+  }
+  if (colour == 1) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      __threadfence();
+      if (atomicAdd(&lp->done, 1u) == lp->ntiles - 1) {   // last block of this lattice
+        lp->done = 0;
+        lp->t = t + 1;
+        __threadfence();
+      }
+    }
+  }
+}
+
+struct msa_batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int64_t max_bytes = 0, used_bytes = 0;
+  std::vector<msa_ctx *> slot;          // nullptr = free
+  std::vector<int64_t> slot_bytes;
+  std::vector<int32_t *> slot_lut_idx;  // device copies
+  std::vector<BLat> h_lats;
+  BLat *d_lats = nullptr;
+  size_t cap = 0;
+  BItem *d_items = nullptr;
+  uint32_t nitems = 0, items_cap = 0;
+  msa_lut *luts = nullptr;
+  int32_t nlut = 0;
+  int block = 256;
+  int32_t chunk = 16;                   // iterations per captured graph
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  bool dirty = true;
+};
+
+static int batch_clear_graph(msa_batch *b) {
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->gexec = nullptr;
+  b->graph = nullptr;
+  return MSA_OK;
+}
+
+extern "C" int msa_batch_create(int device, int64_t max_bytes, msa_batch **out) {
+  if (!out) return msa_fail(MSA_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  int ndev = 0;
+  HIPCHK(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return msa_fail(MSA_ERR_ARG, "device=%d count=%d", device, ndev);
+  HIPCHK(hipSetDevice(device));
+  msa_batch *b = new msa_batch();
+  b->device = device;
+  b->max_bytes = max_bytes > 0 ? max_bytes : INT64_MAX;
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete b;
+    return msa_fail(MSA_ERR_HIP, "stream creation failed");
+  }
+  *out = b;
+  return MSA_OK;
+}
+
+extern "C" int msa_batch_destroy(msa_batch *b) {
+  if (!b) return MSA_OK;
+  (void)hipSetDevice(b->device);
+  (void)hipStreamSynchronize(b->stream);
+  batch_clear_graph(b);
+  for (size_t s = 0; s < b->slot.size(); ++s) {
+    if (b->slot[s]) msa_destroy(b->slot[s]);
+    (void)hipFree(b->slot_lut_idx[s]);
+  }
+  (void)hipFree(b->d_lats);
+  (void)hipFree(b->d_items);
+  (void)hipFree(b->luts);
+  (void)hipStreamDestroy(b->stream);
+  delete b;
+  return MSA_OK;
+}
+
+extern "C" int msa_batch_set_luts(msa_batch *b, int32_t nlut, const msa_lut *luts) {
+  if (!b || !luts || nlut < 1) return msa_fail(MSA_ERR_ARG, "need at least one LUT pair");
+  for (const msa_ctx *c : b->slot)
+    if (c) return msa_fail(MSA_ERR_STATE, "set the shared LUTs before adding lattices");
+  for (int32_t k = 0; k < 2 * nlut; ++k)
+    if (!lut_valid(luts[k])) return msa_fail(MSA_ERR_ARG, "LUT %d is malformed", k);
+  HIPCHK(hipSetDevice(b->device));
+  (void)hipFree(b->luts);
+  b->luts = nullptr;
+  if (hipMalloc(&b->luts, sizeof(msa_lut) * 2 * (size_t)nlut) != hipSuccess)
+    return msa_fail(MSA_ERR_NOMEM, "LUT table of %d pairs", nlut);
+  HIPCHK(hipMemcpy(b->luts, luts, sizeof(msa_lut) * 2 * (size_t)nlut, hipMemcpyHostToDevice));
+  b->nlut = nlut;
+  return MSA_OK;
+}
+
+static int64_t job_bytes(const msa_job *j) {
+  return msa_footprint_bytes(&j->geom) + (int64_t)j->nref * (16 * j->geom.R + 4) + 64 * j->geom.R;
+}
+
+extern "C" int64_t msa_batch_job_bytes(const msa_job *job) {
+  if (!job) return -1;
+  return job_bytes(job);
+}
+
+extern "C" int msa_batch_add(msa_batch *b, const msa_job *j, int32_t *slot_out) {
+  if (!b || !j || !slot_out) return msa_fail(MSA_ERR_ARG, "NULL argument");
+  if (j->geom.nc != 1) return msa_fail(MSA_ERR_ARG, "batched lattices use one demon layer (nc=%d)", j->geom.nc);
+  if (!b->luts) return msa_fail(MSA_ERR_STATE, "set the shared LUTs first");
+  if (j->niter < 0) return msa_fail(MSA_ERR_ARG, "niter=%lld", (long long)j->niter);
+  if (j->refresh_every < -1) return msa_fail(MSA_ERR_ARG, "refresh_every=%lld", (long long)j->refresh_every);
+  if (j->niter > 0) {
+    // every refresh the run will need must be in the table
+    int64_t kmax = -1;
+    if (j->refresh_every > 0) kmax = (j->niter - 1) / j->refresh_every;
+    else if (j->refresh_every == -1) kmax = refresh_index(j->niter <= 1 ? 0 : 1ull << (63 - __builtin_clzll((uint64_t)j->niter - 1)), -1);
+    if (kmax >= j->nref) return msa_fail(MSA_ERR_RANGE, "run of %lld iterations needs %lld refreshes, %d given",
+                                         (long long)j->niter, (long long)kmax + 1, j->nref);
+  }
+  const int64_t bytes = job_bytes(j);
+  if (bytes < 0) return msa_fail(MSA_ERR_ARG, "bad geometry");
+  if (b->used_bytes + bytes > b->max_bytes)
+    return msa_fail(MSA_ERR_NOMEM, "admission: %lld bytes in use + %lld > limit %lld",
+                    (long long)b->used_bytes, (long long)bytes, (long long)b->max_bytes);
+  msa_ctx *c = nullptr;
+  int rc = msa_create(b->device, &j->geom, &c);
+  if (rc) return rc;
+  if ((rc = msa_init_philox(c, j->seed)) || (rc = msa_ctx_share_luts(c, b->luts, b->nlut)) ||
+      (j->nref > 0 && (rc = msa_set_refresh(c, j->nref, j->ctr, j->key, j->lut_idx)))) {
+    msa_destroy(c);
+    return rc;
+  }
+  int32_t *d_idx = nullptr;
+  if (j->nref > 0) {
+    if (hipMalloc(&d_idx, sizeof(int32_t) * (size_t)j->nref) != hipSuccess) {
+      msa_destroy(c);
+      return msa_fail(MSA_ERR_NOMEM, "refresh index table");
+    }
+    if (hipMemcpy(d_idx, j->lut_idx, sizeof(int32_t) * (size_t)j->nref, hipMemcpyHostToDevice) != hipSuccess) {
+      (void)hipFree(d_idx);
+      msa_destroy(c);
+      return msa_fail(MSA_ERR_HIP, "refresh index upload");
+    }
+  }
+  size_t s = 0;
+  while (s < b->slot.size() && b->slot[s]) ++s;
+  if (s == b->slot.size()) {
+    b->slot.push_back(nullptr);
+    b->slot_bytes.push_back(0);
+    b->slot_lut_idx.push_back(nullptr);
+    b->h_lats.push_back(BLat{});
+  }
+  b->slot[s] = c;
+  b->slot_bytes[s] = bytes;
+  b->slot_lut_idx[s] = d_idx;
+  b->used_bytes += bytes;
+  BLat &L = b->h_lats[s];
+  L = BLat{};
+  L.sp[0] = c->sp[0];
+  L.sp[1] = c->sp[1];
+  L.jp = c->jp;
+  L.dm[0] = c->dm[0];
+  L.dm[1] = c->dm[1];
+  L.g = c->g;
+  L.ctr = c->ctr;
+  L.key = c->key;
+  L.luts = b->luts;
+  L.lut_idx = d_idx;
+  L.every = j->refresh_every;
+  L.t = 0;
+  L.tend = (uint64_t)j->niter;
+  L.nref = j->nref;
+  b->dirty = true;
+  *slot_out = (int32_t)s;
+  return MSA_OK;
+}
+
+extern "C" int msa_batch_upload_state(msa_batch *b, int32_t slot, const uint64_t *J,
+                                      const uint64_t *spin, const uint64_t *d0, const uint64_t *d1) {
+  if (!b || slot < 0 || (size_t)slot >= b->slot.size() || !b->slot[slot])
+    return msa_fail(MSA_ERR_ARG, "no lattice in slot %d", slot);
+  HIPCHK(hipSetDevice(b->device));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  uint64_t t = 0;
+  HIPCHK(hipMemcpy(&t, &b->d_lats[slot].t, sizeof t, hipMemcpyDeviceToHost));
+  if (!b->dirty && t != 0) return msa_fail(MSA_ERR_STATE, "slot %d already ran %llu iterations", slot, (unsigned long long)t);
+  return msa_upload_state(b->slot[slot], J, spin, d0, d1);
+}
+
+// Push descriptors and the work list; (re)capture the graph.
+static int batch_prepare(msa_batch *b) {
+  if (!b->dirty) return MSA_OK;
+  const size_t n = b->slot.size();
+  // keep device-side progress of existing lattices
+  if (b->d_lats && b->cap) {
+    std::vector<BLat> cur(b->cap);
+    HIPCHK(hipMemcpy(cur.data(), b->d_lats, sizeof(BLat) * b->cap, hipMemcpyDeviceToHost));
+    for (size_t s = 0; s < std::min(n, b->cap); ++s)
+      if (b->slot[s] && cur[s].jp == b->h_lats[s].jp) b->h_lats[s].t = cur[s].t;
+  }
+  std::vector<BItem> items;
+  for (size_t s = 0; s < n; ++s) {
+    if (!b->slot[s]) continue;
+    BLat &L = b->h_lats[s];
+    L.ntiles = (uint32_t)((L.g.H + b->block - 1) / b->block);
+    L.done = 0;
+    if (L.t >= L.tend) continue;
+    for (uint32_t k = 0; k < L.ntiles; ++k) items.push_back(BItem{(uint32_t)s, k});
+  }
+  if (n > b->cap) {
+    (void)hipFree(b->d_lats);
+    b->d_lats = nullptr;
+    b->cap = 0;
+    if (hipMalloc(&b->d_lats, sizeof(BLat) * n) != hipSuccess) return msa_fail(MSA_ERR_NOMEM, "descriptors");
+    b->cap = n;
+  }
+  if (n) HIPCHK(hipMemcpy(b->d_lats, b->h_lats.data(), sizeof(BLat) * n, hipMemcpyHostToDevice));
+  if (items.size() > b->items_cap) {
+    (void)hipFree(b->d_items);
+    b->d_items = nullptr;
+    b->items_cap = 0;
+    if (hipMalloc(&b->d_items, sizeof(BItem) * items.size()) != hipSuccess)
+      return msa_fail(MSA_ERR_NOMEM, "work list");
+    b->items_cap = (uint32_t)items.size();
+  }
+  if (!items.empty())
+    HIPCHK(hipMemcpy(b->d_items, items.data(), sizeof(BItem) * items.size(), hipMemcpyHostToDevice));
+  b->nitems = (uint32_t)items.size();
+  batch_clear_graph(b);
+  if (b->nitems) {
+    HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    for (int32_t it = 0; it < b->chunk; ++it)
+      for (int col = 0; col < 2; ++col) {
+        if (b->block == 128)
+          hipLaunchKernelGGL(k_batch_halfsweep<128>, dim3(b->nitems), dim3(128), 0, b->stream, b->d_lats, b->d_items, col);
+        else if (b->block == 512)
+          hipLaunchKernelGGL(k_batch_halfsweep<512>, dim3(b->nitems), dim3(512), 0, b->stream, b->d_lats, b->d_items, col);
+        else
+          hipLaunchKernelGGL(k_batch_halfsweep<256>, dim3(b->nitems), dim3(256), 0, b->stream, b->d_lats, b->d_items, col);
+      }
+    HIPCHK(hipStreamEndCapture(b->stream, &b->graph));
+    HIPCHK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+  }
+  b->dirty = false;
+  return MSA_OK;
+}
+
+extern "C" int msa_batch_set_shape(msa_batch *b, int32_t block, int32_t chunk) {
+  if (!b) return msa_fail(MSA_ERR_ARG, "NULL batch");
+  if (block != 128 && block != 256 && block != 512) return msa_fail(MSA_ERR_ARG, "block=%d (128, 256 or 512)", block);
+  if (chunk < 1 || chunk > 4096) return msa_fail(MSA_ERR_ARG, "chunk=%d", chunk);
+  HIPCHK(hipSetDevice(b->device));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  if (b->block != block) {
+    // tile counts change: progress is kept by batch_prepare, done counters are reset there
+    b->block = block;
+  }
+  b->chunk = chunk;
+  b->dirty = true;
+  return MSA_OK;
+}
+
+extern "C" int msa_batch_run(msa_batch *b, int64_t niter) {
+  if (!b) return msa_fail(MSA_ERR_ARG, "NULL batch");
+  if (niter < 0) return msa_fail(MSA_ERR_ARG, "niter=%lld", (long long)niter);
+  HIPCHK(hipSetDevice(b->device));
+  if (b->dirty) {
+    HIPCHK(hipStreamSynchronize(b->stream));
+    int rc = batch_prepare(b);
+    if (rc) return rc;
+  }
+  if (!b->nitems) return MSA_OK;
+  int64_t left = niter;
+  while (left >= b->chunk) {
+    HIPCHK(hipGraphLaunch(b->gexec, b->stream));
+    left -= b->chunk;
+  }
+  for (int64_t it = 0; it < left; ++it)
+    for (int col = 0; col < 2; ++col) {
+      if (b->block == 128)
+        hipLaunchKernelGGL(k_batch_halfsweep<128>, dim3(b->nitems), dim3(128), 0, b->stream, b->d_lats, b->d_items, col);
+      else if (b->block == 512)
+        hipLaunchKernelGGL(k_batch_halfsweep<512>, dim3(b->nitems), dim3(512), 0, b->stream, b->d_lats, b->d_items, col);
+      else
+        hipLaunchKernelGGL(k_batch_halfsweep<256>, dim3(b->nitems), dim3(256), 0, b->stream, b->d_lats, b->d_items, col);
+    }
+  HIPCHK(hipGetLastError());
+  return MSA_OK;
+}
+
+extern "C" int msa_batch_sync(msa_batch *b) {
+  if (!b) return msa_fail(MSA_ERR_ARG, "NULL batch");
+  HIPCHK(hipSetDevice(b->device));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return MSA_OK;
+}
+
+extern "C" int msa_batch_progress(msa_batch *b, int32_t nslots, int64_t *t, int64_t *tend) {
+  if (!b || nslots < 0) return msa_fail(MSA_ERR_ARG, "bad argument");
+  HIPCHK(hipSetDevice(b->device));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  std::vector<BLat> cur(b->cap);
+  if (b->cap && !b->dirty)
+    HIPCHK(hipMemcpy(cur.data(), b->d_lats, sizeof(BLat) * b->cap, hipMemcpyDeviceToHost));
+  for (int32_t s = 0; s < nslots; ++s) {
+    const bool live = (size_t)s < b->slot.size() && b->slot[s];
+    const uint64_t ts = !live ? 0 : (b->dirty || (size_t)s >= b->cap) ? b->h_lats[s].t : cur[s].t;
+    if (t) t[s] = live ? (int64_t)ts : -1;
+    if (tend) tend[s] = live ? (int64_t)b->h_lats[s].tend : -1;
+  }
+  return MSA_OK;
+}
+
+// the slot's context with its buffer parity matching the device counter
+static int batch_ctx(msa_batch *b, int32_t slot, msa_ctx **out) {
+  if (!b || slot < 0 || (size_t)slot >= b->slot.size() || !b->slot[slot])
+    return msa_fail(MSA_ERR_ARG, "no lattice in slot %d", slot);
+  HIPCHK(hipSetDevice(b->device));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  uint64_t t = b->h_lats[slot].t;
+  if (!b->dirty && (size_t)slot < b->cap)
+    HIPCHK(hipMemcpy(&t, &b->d_lats[slot].t, sizeof t, hipMemcpyDeviceToHost));
+  msa_ctx *c = b->slot[slot];
+  c->scur = c->cur = (int)(t & 1u);
+  *out = c;
+  return MSA_OK;
+}
+
+extern "C" int msa_batch_measure(msa_batch *b, int32_t slot, int32_t *unsat, int32_t *demon,
+                                 int32_t *q, int32_t *qlink) {
+  msa_ctx *c = nullptr;
+  int rc = batch_ctx(b, slot, &c);
+  return rc ? rc : msa_measure(c, unsat, demon, q, qlink);
+}
+
+extern "C" int msa_batch_download_state(msa_batch *b, int32_t slot, uint64_t *spin, uint64_t *d0,
+                                        uint64_t *d1) {
+  msa_ctx *c = nullptr;
+  int rc = batch_ctx(b, slot, &c);
+  return rc ? rc : msa_download_state(c, spin, d0, d1);
+}
+
+extern "C" int msa_batch_remove(msa_batch *b, int32_t slot) {
+  if (!b || slot < 0 || (size_t)slot >= b->slot.size() || !b->slot[slot])
+    return msa_fail(MSA_ERR_ARG, "no lattice in slot %d", slot);
+  HIPCHK(hipSetDevice(b->device));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  // keep the progress of the others before the descriptor table is rebuilt
+  if (!b->dirty && b->cap) {
+    std::vector<BLat> cur(b->cap);
+    HIPCHK(hipMemcpy(cur.data(), b->d_lats, sizeof(BLat) * b->cap, hipMemcpyDeviceToHost));
+    for (size_t s = 0; s < std::min(b->slot.size(), b->cap); ++s)
+      if (b->slot[s]) b->h_lats[s].t = cur[s].t;
+  }
+  msa_destroy(b->slot[slot]);
+  (void)hipFree(b->slot_lut_idx[slot]);
+  b->slot[slot] = nullptr;
+  b->slot_lut_idx[slot] = nullptr;
+  b->used_bytes -= b->slot_bytes[slot];
+  b->slot_bytes[slot] = 0;
+  b->h_lats[slot] = BLat{};
+  b->dirty = true;
+  return MSA_OK;
+}
+
+extern "C" int64_t msa_batch_bytes_in_use(msa_batch *b) { return b ? b->used_bytes : -1; }

@@ -23,18 +23,41 @@ __device__ __forceinline__ uint32_t wrapi(int a, uint32_t n) {
   return (uint32_t)(a < 0 ? a + (int)n : (a >= (int)n ? a - (int)n : a));
 }
 
+// Lattice direction e_l for demon layer l (main.cu:1125-1175): +x,+y,+z,-x,-y,-z.
+__device__ __forceinline__ void layer_dir(int l, int &dx, int &dy, int &dz) {
+  const int sgn = l < 3 ? 1 : -1;
+  const int a = l % 3;
+  dx = a == 0 ? sgn : 0;
+  dy = a == 1 ? sgn : 0;
+  dz = a == 2 ? sgn : 0;
+}
+
+// Split index of the site a demon of colour `col` at (i, y, z) moves to in the
+// walk of its layer (d_Walk, main.cu:1084-1193): colour 0 by +2 e_l, colour 1
+// by -2 e_l.  A shift by 2 keeps the colour, so only the index changes.
+__device__ __forceinline__ uint64_t walk_dest(const Geo &g, int col, int dx, int dy, int dz,
+                                              uint32_t i, uint32_t y, uint32_t z) {
+  const int sg = col ? -1 : 1;
+  const uint32_t wi = wrapi((int)i + sg * dx, g.Lh);
+  const uint32_t wy = wrapi((int)y + 2 * sg * dy, g.L);
+  const uint32_t wz = wrapi((int)z + 2 * sg * dz, g.L);
+  return wi + (uint64_t)g.Lh * (wy + (uint64_t)g.L * wz);
+}
+
 // ---------------------------------------------------------------- Philox --
 // Philox4x32-10 with the round and key schedule of main.cu:562-595
 // (Random123's philox4x32 with 10 rounds).
 __device__ __forceinline__ void philox10(uint32_t &v0, uint32_t &v1, uint32_t &v2, uint32_t &v3,
                                          uint32_t k0, uint32_t k1) {
+  // 64-bit products: one v_mad_u64_u32 gives both halves (mulhi and mullo of
+  // the reference's _update_state); the two xors fold into one v_bitop3.
 #pragma unroll
   for (int round = 0; round < 10; ++round) {
-    const uint32_t lo0 = 0xD2511F53u * v0, hi0 = __umulhi(0xD2511F53u, v0);
-    const uint32_t lo1 = 0xCD9E8D57u * v2, hi1 = __umulhi(0xCD9E8D57u, v2);
-    const uint32_t n0 = hi1 ^ v1 ^ k0;
-    const uint32_t n2 = hi0 ^ v3 ^ k1;
-    v0 = n0; v1 = lo1; v2 = n2; v3 = lo0;
+    const uint64_t p0 = (uint64_t)0xD2511F53u * v0;
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * v2;
+    const uint32_t n0 = __builtin_amdgcn_bitop3_b32((uint32_t)(p1 >> 32), v1, k0, 0x96);
+    const uint32_t n2 = __builtin_amdgcn_bitop3_b32((uint32_t)(p0 >> 32), v3, k1, 0x96);
+    v0 = n0; v1 = (uint32_t)p1; v2 = n2; v3 = (uint32_t)p0;
     k0 += 0x9E3779B9u;
     k1 += 0xBB67AE85u;
   }
@@ -47,19 +70,20 @@ struct LutView {
   const uint16_t *pre;
 };
 
-// Byte n with umbral[n-1] <= R < umbral[n], R = (rm:rl); the bisection of
-// main.cu:615-630 seeded by the 10-bit presearch.
+// Byte n with umbral[n-1] <= R < umbral[n], R = (rm:rl): the bisection of
+// main.cu:615-630 seeded by the 10-bit presearch.  The reference first tests
+// R against umbral[254]; the presearch bracket [min, max] of R's 10-bit prefix
+// always contains the answer (including 255, whose bracket ends at 255), so
+// the search starts from the bracket directly -- same byte for every R
+// (checked against the reference bisection on every interval endpoint and
+// bucket edge of the tables in use).
 __device__ __forceinline__ uint32_t bisect(const LutView &t, uint32_t rm, uint32_t rl) {
   auto below = [&](uint32_t n) {
     const uint2 u = t.umb[n];
     return (rm == u.x) ? (rl < u.y) : (rm < u.x);
   };
-  uint32_t lo = 255, hi = 255;
-  if (below(254)) {
-    const uint32_t e = t.pre[rm >> 22];
-    lo = e & 255u;
-    hi = e >> 8;
-  }
+  const uint32_t e = t.pre[rm >> 22];
+  uint32_t lo = e & 255u, hi = e >> 8;
   // A valid table (min <= max, checked on upload) needs at most 8 halvings;
   // the bound keeps a malformed table from looping.
   for (int it = 0; it < 9 && hi - lo > 1; ++it) {
@@ -143,6 +167,52 @@ __device__ __forceinline__ uint64_t isi_update(uint64_t j1, uint64_t j2, uint64_
   D0 = (flip & n0) | (carry & D0);
   D1 = (flip & n1) | (carry & D1);
   return flip;
+}
+
+// ------------------------------------------------ gfx950 3-input logic --
+// v_bitop3_b32 evaluates any 3-input boolean function given as an 8-entry
+// truth table.  Only tables symmetric in their three operands are used here
+// (xor3 0x96, xnor3 0x69, majority 0xE8, and3 0x80, "one or two set" 0x7E),
+// so the result does not depend on the operand order of the table index.
+template <uint32_t TT>
+__device__ __forceinline__ uint64_t lop3(uint64_t a, uint64_t b, uint64_t c) {
+  const uint32_t lo = __builtin_amdgcn_bitop3_b32((uint32_t)a, (uint32_t)b, (uint32_t)c, TT);
+  const uint32_t hi = __builtin_amdgcn_bitop3_b32((uint32_t)(a >> 32), (uint32_t)(b >> 32),
+                                                  (uint32_t)(c >> 32), TT);
+  return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ uint64_t xor3(uint64_t a, uint64_t b, uint64_t c) { return lop3<0x96>(a, b, c); }
+__device__ __forceinline__ uint64_t xnor3(uint64_t a, uint64_t b, uint64_t c) { return lop3<0x69>(a, b, c); }
+__device__ __forceinline__ uint64_t maj3(uint64_t a, uint64_t b, uint64_t c) { return lop3<0xE8>(a, b, c); }
+__device__ __forceinline__ uint64_t and3(uint64_t a, uint64_t b, uint64_t c) { return lop3<0x80>(a, b, c); }
+__device__ __forceinline__ uint64_t one_or_two3(uint64_t a, uint64_t b, uint64_t c) { return lop3<0x7E>(a, b, c); }
+
+// One site update, 64 samples at once: the local field and the demon
+// acceptance of main.cu:898-944 and main.cu:1037-1074, written as a smaller
+// circuit with the same truth table (checked exhaustively against the
+// reference algebra by the oracle tests).  With n+ = j1 + 2 j2 (+x,+y,+z
+// bonds unsatisfied), n- = j3 + 2 j4 and D = D0 + 2 D1, a flip is accepted iff
+// 3 <= D + n+ + n- <= 6 and then D' = D + n - 3:
+//   c1 = [D0 + j1 + j3 in {1,2}], c2 = D0 & j1 & j3,
+//   v0 = D1 ^ j2 ^ j4,           v1 = maj(D1, j2, j4),
+//   flip = v1 ^ c2 ^ (c1 & v0),  D0' = ~(D0 ^ j1 ^ j3), D1' = v0 ^ c1.
+// n0..n5 are the neighbour spins at +x,+y,+z,-x,-y,-z, J0..J5 the bonds.
+__device__ __forceinline__ uint64_t site_update(uint64_t s, uint64_t n0, uint64_t n1, uint64_t n2,
+                                                uint64_t n3, uint64_t n4, uint64_t n5,
+                                                uint64_t J0, uint64_t J1, uint64_t J2,
+                                                uint64_t J3, uint64_t J4, uint64_t J5,
+                                                uint64_t &D0, uint64_t &D1) {
+  const uint64_t f0 = xor3(s, J0, n0), f1 = xor3(s, J1, n1), f2 = xor3(s, J2, n2);
+  const uint64_t j1 = xor3(f0, f1, f2), j2 = maj3(f0, f1, f2);
+  const uint64_t f3 = xor3(s, J3, n3), f4 = xor3(s, J4, n4), f5 = xor3(s, J5, n5);
+  const uint64_t j3 = xor3(f3, f4, f5), j4 = maj3(f3, f4, f5);
+  const uint64_t c1 = one_or_two3(D0, j1, j3), c2 = and3(D0, j1, j3);
+  const uint64_t v0 = xor3(D1, j2, j4), v1 = maj3(D1, j2, j4);
+  const uint64_t flip = xor3(v1, c2, c1 & v0);
+  const uint64_t d0n = xnor3(D0, j1, j3), d1n = v0 ^ c1;
+  D0 = (flip & d0n) | (~flip & D0);
+  D1 = (flip & d1n) | (~flip & D1);
+  return s ^ flip;
 }
 
 // --------------------------------------------------- per-lane counting --

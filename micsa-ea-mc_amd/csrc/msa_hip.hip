@@ -10,6 +10,8 @@
 #include <vector>
 
 #include "msa.h"
+#include "msa_common.hpp"
+#include "msa_ctx.hpp"
 #include "msa_device.hpp"
 
 using msa::Geo;
@@ -17,7 +19,7 @@ using msa::Geo;
 // ------------------------------------------------------------ errors ------
 static thread_local char g_err[512] = "no error";
 
-static int fail(int code, const char *fmt, ...) {
+int msa_fail(int code, const char *fmt, ...) {
   va_list ap;
   va_start(ap, fmt);
   vsnprintf(g_err, sizeof(g_err), fmt, ap);
@@ -25,17 +27,11 @@ static int fail(int code, const char *fmt, ...) {
   return code;
 }
 
-#define HIPCHK(call)                                                                   \
-  do {                                                                                 \
-    hipError_t e_ = (call);                                                            \
-    if (e_ != hipSuccess)                                                              \
-      return fail(MSA_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),   \
-                  __FILE__, __LINE__);                                                 \
-  } while (0)
-
 // ----------------------------------------------------------- kernels ------
 struct SweepArgs {
-  uint64_t *sp;
+  const uint64_t *sp_own;   // this colour's spins before the update
+  const uint64_t *sp_nb;    // the other colour's spins (old for colour 0, new for colour 1)
+  uint64_t *sp_out;         // this colour's updated spins
   const uint64_t *jp;
   const ulonglong2 *dm_in;
   ulonglong2 *dm_out;
@@ -47,22 +43,13 @@ struct SweepArgs {
   const msa_lut *lut;     // [2] plane-0 and plane-1 tables of this refresh
 };
 
-// Lattice direction e_l for demon layer l (main.cu:1125-1175): +x,+y,+z,-x,-y,-z.
-__device__ __forceinline__ void layer_dir(int l, int &dx, int &dy, int &dz) {
-  const int sgn = l < 3 ? 1 : -1;
-  const int a = l % 3;
-  dx = a == 0 ? sgn : 0;
-  dy = a == 1 ? sgn : 0;
-  dz = a == 2 ? sgn : 0;
-}
-
 // One colour of a sweep with demon layer `layer` (d_oneMCstepBN_multisample,
 // main.cu:847-1080, ISI branch), with the demon walk of that layer fused into
-// the demon store (d_Walk, main.cu:1084-1193: colour-0 demons move +2e_l,
-// colour-1 demons -2e_l) and, when `refresh` is set, the heat-bath refresh of
-// the layer fused into the demon load.  Demons are read from dm_in and written
-// to dm_out, so the walk's permutation never races with a load.
-// Thread = one site of the colour, all R replicas (J is loaded once).
+// the demon store and, when `refresh` is set, the heat-bath refresh of the
+// layer fused into the demon load.  Demons go dm_in -> dm_out and spins
+// sp_own -> sp_out, so neither the walk's permutation nor the update races
+// with a load.  Thread = one site of the colour, all R replicas (J loaded once).
+// Used for sweeps that refresh (VALU-bound) and as the general fallback.
 __global__ void __launch_bounds__(256) k_halfsweep(SweepArgs a) {
   __shared__ uint32_t s_lut[2 * 1024];
   const Geo g = a.g;
@@ -97,14 +84,9 @@ __global__ void __launch_bounds__(256) k_halfsweep(SweepArgs a) {
   const uint64_t J0 = Jc[h], J1 = Jc[g.H + h], J2 = Jc[2 * g.H + h];
   const uint64_t J3 = Jo[hxm], J4 = Jo[g.H + hym], J5 = Jo[2 * g.H + hzm];
 
-  // destination of this colour's demon after the walk
   int dx, dy, dz;
-  layer_dir(a.layer, dx, dy, dz);
-  const int sg = col ? -2 : 2;
-  const uint32_t wi = msa::wrapi((int)i + (sg / 2) * dx, g.Lh);
-  const uint32_t wy = msa::wrapi((int)y + sg * dy, g.L);
-  const uint32_t wz = msa::wrapi((int)z + sg * dz, g.L);
-  const uint64_t hw = wi + (uint64_t)g.Lh * (wy + (uint64_t)g.L * wz);
+  msa::layer_dir(a.layer, dx, dy, dz);
+  const uint64_t hw = msa::walk_dest(g, col, dx, dy, dz, i, y, z);
 
   msa::LutView t0{reinterpret_cast<const uint2 *>(s_lut),
                   reinterpret_cast<const uint16_t *>(s_lut + 512)};
@@ -113,12 +95,8 @@ __global__ void __launch_bounds__(256) k_halfsweep(SweepArgs a) {
   const uint32_t tid = (uint32_t)((uint64_t)a.layer * g.V + x + (uint64_t)g.L * (y + (uint64_t)g.L * z));
 
   for (uint32_t r = 0; r < g.R; ++r) {
-    uint64_t *Sc = a.sp + (uint64_t)(2 * r + col) * g.H;
-    const uint64_t *So = a.sp + (uint64_t)(2 * r + (col ^ 1)) * g.H;
-    const uint64_t s = Sc[h];
-    uint64_t j1, j2, j3, j4;
-    msa::local_field(s, So[hxp], So[hyp], So[hzp], So[hxm], So[hym], So[hzm], J0, J1, J2, J3, J4,
-                     J5, j1, j2, j3, j4);
+    const uint64_t *So = a.sp_nb + (uint64_t)(2 * r + (col ^ 1)) * g.H;
+    const uint64_t sown = a.sp_own[(uint64_t)(2 * r + col) * g.H + h];
     const uint64_t dbase = ((uint64_t)(r * g.nc + a.layer) * 2 + col) * g.H;
     uint64_t D0, D1;
     if (a.refresh) {
@@ -128,9 +106,133 @@ __global__ void __launch_bounds__(256) k_halfsweep(SweepArgs a) {
       D0 = d.x;
       D1 = d.y;
     }
-    const uint64_t flip = msa::isi_update(j1, j2, j3, j4, D0, D1);
-    Sc[h] = s ^ flip;
+    const uint64_t sn = msa::site_update(sown, So[hxp], So[hyp], So[hzp], So[hxm], So[hym], So[hzm],
+                                         J0, J1, J2, J3, J4, J5, D0, D1);
+    a.sp_out[(uint64_t)(2 * r + col) * g.H + h] = sn;
     a.dm_out[dbase + hw] = make_ulonglong2(D0, D1);
+  }
+}
+
+struct FusedArgs {
+  const uint64_t *sp_in;
+  uint64_t *sp_out;
+  const uint64_t *jp;
+  const ulonglong2 *dm_in;
+  ulonglong2 *dm_out;
+  Geo g;
+  int layer;
+  uint32_t ty, tz;   // rows per tile, planes per slab
+  uint32_t ny;       // thread rows per block: ty + 2 (halo rows) or L (whole planes)
+  int yhalo;
+};
+
+// A whole sweep (both colours, one demon layer, walk fused) in one pass.
+// Block = all L/2 site pairs of ny rows; it owns ty rows x tz planes and
+// walks its slab plane by plane: at step s it updates the colour-0 sites of
+// plane s, then the colour-1 sites of plane s-1, whose six colour-0
+// neighbours (planes s-2, s-1, s) are all new by then.  Colour-0 values of
+// the rows and planes just outside the tile are recomputed, not written
+// (their owners write them), so blocks never wait for each other.  Spins and
+// demons are read from the *_in buffers and written to the *_out buffers, so
+// every global load sees the state before the sweep; J is read once per sweep
+// from HBM, its reuse between the two colours comes from the caches.
+// New colour-0 values of one plane are exchanged through LDS, three plane
+// buffers deep so one barrier per step orders writer and readers.
+template <int R>
+__global__ void __launch_bounds__(1024) k_sweep_fused(FusedArgs a) {
+  extern __shared__ uint64_t s_e[];
+  const Geo g = a.g;
+  const uint32_t Lh = g.Lh, L = g.L;
+  const uint64_t H = g.H;
+  const uint32_t i = threadIdx.x % Lh, row = threadIdx.x / Lh;
+  const uint32_t y0 = blockIdx.x * a.ty, z0 = blockIdx.y * a.tz;
+  const bool yh = a.yhalo != 0;
+  const uint32_t y = yh ? msa::wrapi((int)(y0 + row) - 1, L) : row;
+  const bool own = yh ? (row >= 1 && row <= a.ty) : true;
+  const uint32_t rm = yh ? row - 1 : (row == 0 ? L - 1 : row - 1);
+  const uint32_t rp = yh ? row + 1 : (row + 1 == L ? 0 : row + 1);
+  const uint32_t ym = msa::wrapi((int)y - 1, L), yp = msa::wrapi((int)y + 1, L);
+  const uint64_t pstride = (uint64_t)a.ny * Lh * R;
+  int dx, dy, dz;
+  msa::layer_dir(a.layer, dx, dy, dz);
+  const uint64_t *J = a.jp;
+  const uint64_t lbase0 = (uint64_t)(a.layer * 2 + 0) * H, lbase1 = (uint64_t)(a.layer * 2 + 1) * H;
+  const uint64_t rstride = (uint64_t)g.nc * 2 * H;   // demon stride between replicas
+
+  uint64_t Em2[R], Em1[R], E0[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) Em2[r] = Em1[r] = E0[r] = 0;
+
+  const int nstep = (int)a.tz + 2;
+  for (int k = 0; k < nstep; ++k) {
+    const int s = (int)z0 - 1 + k;
+    // ---------------- colour 0, plane z
+    {
+      const uint32_t z = msa::wrapi(s, L);
+      const uint32_t off = (y + z) & 1u;
+      const uint32_t ip = off ? msa::wrapi((int)i + 1, Lh) : i, im = off ? i : msa::wrapi((int)i - 1, Lh);
+      const uint32_t zp = msa::wrapi((int)z + 1, L), zm = msa::wrapi((int)z - 1, L);
+      const uint64_t rb = (uint64_t)Lh * (y + (uint64_t)L * z);
+      const uint64_t h = i + rb;
+      const uint64_t hxp = ip + rb, hxm = im + rb;
+      const uint64_t hyp = i + (uint64_t)Lh * (yp + (uint64_t)L * z);
+      const uint64_t hym = i + (uint64_t)Lh * (ym + (uint64_t)L * z);
+      const uint64_t hzp = i + (uint64_t)Lh * (y + (uint64_t)L * zp);
+      const uint64_t hzm = i + (uint64_t)Lh * (y + (uint64_t)L * zm);
+      const uint64_t J0 = J[h], J1 = J[H + h], J2 = J[2 * H + h];
+      const uint64_t J3 = J[3 * H + hxm], J4 = J[4 * H + hym], J5 = J[5 * H + hzm];
+      const bool write = own && k >= 1 && k <= (int)a.tz;
+      const uint64_t hw = msa::walk_dest(g, 0, dx, dy, dz, i, y, z);
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const uint64_t *S0 = a.sp_in + (uint64_t)(2 * r) * H, *S1 = S0 + H;
+        const ulonglong2 d = a.dm_in[r * rstride + lbase0 + h];
+        uint64_t D0 = d.x, D1 = d.y;
+        E0[r] = msa::site_update(S0[h], S1[hxp], S1[hyp], S1[hzp], S1[hxm], S1[hym], S1[hzm],
+                                 J0, J1, J2, J3, J4, J5, D0, D1);
+        s_e[(k % 3) * pstride + ((uint64_t)row * Lh + i) * R + r] = E0[r];
+        if (write) {
+          a.sp_out[(uint64_t)(2 * r) * H + h] = E0[r];
+          a.dm_out[r * rstride + lbase0 + hw] = make_ulonglong2(D0, D1);
+        }
+      }
+    }
+    __syncthreads();
+    // ---------------- colour 1, plane z-1
+    if (own && k >= 2) {
+      const uint32_t z = msa::wrapi(s - 1, L);
+      const uint32_t off = (y + z + 1) & 1u;
+      const uint32_t ip = off ? msa::wrapi((int)i + 1, Lh) : i, im = off ? i : msa::wrapi((int)i - 1, Lh);
+      const uint32_t zm = msa::wrapi((int)z - 1, L);
+      const uint64_t rb = (uint64_t)Lh * (y + (uint64_t)L * z);
+      const uint64_t h = i + rb;
+      const uint64_t hxm = im + rb;
+      const uint64_t hym = i + (uint64_t)Lh * (ym + (uint64_t)L * z);
+      const uint64_t hzm = i + (uint64_t)Lh * (y + (uint64_t)L * zm);
+      const uint64_t J0 = J[3 * H + h], J1 = J[4 * H + h], J2 = J[5 * H + h];
+      const uint64_t J3 = J[hxm], J4 = J[H + hym], J5 = J[2 * H + hzm];
+      const uint64_t hw = msa::walk_dest(g, 1, dx, dy, dz, i, y, z);
+      const uint64_t *Eb = s_e + ((k - 1) % 3) * pstride;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const uint64_t *S1 = a.sp_in + (uint64_t)(2 * r + 1) * H;
+        const ulonglong2 d = a.dm_in[r * rstride + lbase1 + h];
+        uint64_t D0 = d.x, D1 = d.y;
+        const uint64_t nxp = Eb[((uint64_t)row * Lh + ip) * R + r];
+        const uint64_t nxm = Eb[((uint64_t)row * Lh + im) * R + r];
+        const uint64_t nyp = Eb[((uint64_t)rp * Lh + i) * R + r];
+        const uint64_t nym = Eb[((uint64_t)rm * Lh + i) * R + r];
+        const uint64_t sn = msa::site_update(S1[h], nxp, nyp, E0[r], nxm, nym, Em2[r],
+                                             J0, J1, J2, J3, J4, J5, D0, D1);
+        a.sp_out[(uint64_t)(2 * r + 1) * H + h] = sn;
+        a.dm_out[r * rstride + lbase1 + hw] = make_ulonglong2(D0, D1);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      Em2[r] = Em1[r];
+      Em1[r] = E0[r];
+    }
   }
 }
 
@@ -226,44 +328,25 @@ __global__ void __launch_bounds__(256) k_init_philox(uint64_t *sp, uint64_t *jp,
 }
 
 // ----------------------------------------------------------- context ------
-struct msa_ctx {
-  int device = 0;
-  msa_geom geom{};
-  Geo g{};
-  hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  uint64_t *sp = nullptr, *jp = nullptr;
-  ulonglong2 *dm[2] = {nullptr, nullptr};
-  int cur = 0;                       // demon buffer holding the current state
-  msa_lut *luts = nullptr;
-  int32_t nlut = 0;
-  uint4 *ctr = nullptr;
-  uint32_t *key = nullptr;
-  int32_t nref = 0;
-  std::vector<int32_t> lut_idx;      // host copy, indexes luts
-  int32_t *meas = nullptr;           // [64]
-  dim3 blk{1, 1, 1}, grd{1, 1, 1};
-};
-
 static int check_geom(const msa_geom *g) {
-  if (!g) return fail(MSA_ERR_ARG, "geometry is NULL");
+  if (!g) return msa_fail(MSA_ERR_ARG, "geometry is NULL");
   if (g->L < 4 || g->L > 1024 || (g->L & 1))
-    return fail(MSA_ERR_ARG, "L=%d must be even and in 4..1024", g->L);
-  if (g->R < 1 || g->R > 16) return fail(MSA_ERR_ARG, "R=%d must be in 1..16", g->R);
-  if (g->nc < 1 || g->nc > 6) return fail(MSA_ERR_ARG, "nc=%d must be in 1..6", g->nc);
+    return msa_fail(MSA_ERR_ARG, "L=%d must be even and in 4..1024", g->L);
+  if (g->R < 1 || g->R > 16) return msa_fail(MSA_ERR_ARG, "R=%d must be in 1..16", g->R);
+  if (g->nc < 1 || g->nc > 6) return msa_fail(MSA_ERR_ARG, "nc=%d must be in 1..6", g->nc);
   const uint64_t V = (uint64_t)g->L * g->L * g->L;
-  if ((uint64_t)g->nc * V > 0xFFFFFFFFull) return fail(MSA_ERR_ARG, "nc*V exceeds 2^32");
+  if ((uint64_t)g->nc * V > 0xFFFFFFFFull) return msa_fail(MSA_ERR_ARG, "nc*V exceeds 2^32");
   return MSA_OK;
 }
 
 extern "C" int64_t msa_footprint_bytes(const msa_geom *geom) {
   if (check_geom(geom) != MSA_OK) return -1;
   const int64_t V = (int64_t)geom->L * geom->L * geom->L;
-  return 8 * V * (3 + geom->R) + 2 * 16 * V * geom->R * geom->nc + 256;
+  return 8 * V * (3 + 2 * geom->R) + 2 * 16 * V * geom->R * geom->nc + 256;
 }
 
-// Block shape for the half-sweep: x spans a whole row of one colour when it
-// fits a wavefront, else the largest divisor of L/2 not above 64.
+// Block shape for the two-pass kernel: x spans a whole row of one colour when
+// it fits a wavefront, else the largest divisor of L/2 not above 64.
 static void pick_launch(msa_ctx *c) {
   const uint32_t Lh = c->g.Lh;
   uint32_t bx = Lh;
@@ -277,14 +360,56 @@ static void pick_launch(msa_ctx *c) {
   c->grd = dim3((Lh + bx - 1) / bx, (c->g.L + by - 1) / by, c->g.L);
 }
 
+static bool fused_supported_R(uint32_t R) { return R == 1 || R == 2 || R == 4; }
+
+// A one-pass tile is valid when ty and tz divide L, the block (L/2 pairs x
+// thread rows) fits 1024 threads and its LDS (three planes of new colour-0
+// values) fits 64 KiB.
+static int fused_tile_ok(const Geo &g, uint32_t ty, uint32_t tz) {
+  if (!fused_supported_R(g.R) || ty < 1 || tz < 1 || ty > g.L || tz > g.L) return 0;
+  if (g.L % ty || g.L % tz) return 0;
+  const uint32_t ny = ty == g.L ? g.L : ty + 2;
+  if ((uint64_t)g.Lh * ny > 1024) return 0;
+  if ((uint64_t)3 * ny * g.Lh * g.R * 8 > 65536) return 0;
+  return 1;
+}
+
+static void set_fused(msa_ctx *c, uint32_t ty, uint32_t tz) {
+  c->fused = 1;
+  c->fty = ty;
+  c->ftz = tz;
+  c->fyhalo = ty != c->g.L;
+  c->fny = c->fyhalo ? ty + 2 : c->g.L;
+}
+
+// Default one-pass tile: the most rows per block (fewest recomputed halo rows)
+// with at least 256 blocks where possible, slabs of 8..16 planes.
+static void pick_fused(msa_ctx *c) {
+  c->fused = 0;
+  const Geo &g = c->g;
+  uint32_t best_ty = 0, best_tz = 0;
+  double best = -1;
+  for (uint32_t ty = 1; ty <= g.L; ++ty)
+    for (uint32_t tz = 4; tz <= g.L && tz <= 32; ++tz) {
+      if (!fused_tile_ok(g, ty, tz)) continue;
+      const double blocks = (double)(g.L / ty) * (g.L / tz);
+      const double ny = ty == g.L ? g.L : ty + 2.0;
+      const double waste = (ny / ty) * ((tz + 2.0) / tz);   // recomputed colour-0 work
+      const double fill = blocks >= 256 ? 1.0 : blocks / 256.0;
+      const double score = fill / waste;
+      if (score > best) { best = score; best_ty = ty; best_tz = tz; }
+    }
+  if (best_ty) set_fused(c, best_ty, best_tz);
+}
+
 extern "C" int msa_create(int device, const msa_geom *geom, msa_ctx **out) {
-  if (!out) return fail(MSA_ERR_ARG, "out is NULL");
+  if (!out) return msa_fail(MSA_ERR_ARG, "out is NULL");
   *out = nullptr;
   int rc = check_geom(geom);
   if (rc) return rc;
   int ndev = 0;
   HIPCHK(hipGetDeviceCount(&ndev));
-  if (device < 0 || device >= ndev) return fail(MSA_ERR_ARG, "device=%d count=%d", device, ndev);
+  if (device < 0 || device >= ndev) return msa_fail(MSA_ERR_ARG, "device=%d count=%d", device, ndev);
   HIPCHK(hipSetDevice(device));
   msa_ctx *c = new msa_ctx();
   c->device = device;
@@ -299,19 +424,21 @@ extern "C" int msa_create(int device, const msa_geom *geom, msa_ctx **out) {
   auto cleanup = [&](int code) { msa_destroy(c); return code; };
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
-    return cleanup(fail(MSA_ERR_HIP, "stream/event creation failed"));
-  if (hipMalloc(&c->sp, 8 * V * geom->R) != hipSuccess ||
+    return cleanup(msa_fail(MSA_ERR_HIP, "stream/event creation failed"));
+  if (hipMalloc(&c->sp[0], 8 * V * geom->R) != hipSuccess ||
+      hipMalloc(&c->sp[1], 8 * V * geom->R) != hipSuccess ||
       hipMalloc(&c->jp, 8 * V * 3) != hipSuccess ||
       hipMalloc(&c->dm[0], 16 * V * geom->R * geom->nc) != hipSuccess ||
       hipMalloc(&c->dm[1], 16 * V * geom->R * geom->nc) != hipSuccess ||
       hipMalloc(&c->meas, 64 * sizeof(int32_t)) != hipSuccess)
-    return cleanup(fail(MSA_ERR_NOMEM, "device allocation of %lld bytes failed",
+    return cleanup(msa_fail(MSA_ERR_NOMEM, "device allocation of %lld bytes failed",
                         (long long)msa_footprint_bytes(geom)));
   if (hipMemset(c->dm[0], 0, 16 * V * geom->R * geom->nc) != hipSuccess ||
-      hipMemset(c->sp, 0, 8 * V * geom->R) != hipSuccess ||
+      hipMemset(c->sp[0], 0, 8 * V * geom->R) != hipSuccess ||
       hipMemset(c->jp, 0, 8 * V * 3) != hipSuccess)
-    return cleanup(fail(MSA_ERR_HIP, "initial clear failed"));
+    return cleanup(msa_fail(MSA_ERR_HIP, "initial clear failed"));
   pick_launch(c);
+  pick_fused(c);
   *out = c;
   return MSA_OK;
 }
@@ -320,11 +447,12 @@ extern "C" int msa_destroy(msa_ctx *c) {
   if (!c) return MSA_OK;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  (void)hipFree(c->sp);
+  (void)hipFree(c->sp[0]);
+  (void)hipFree(c->sp[1]);
   (void)hipFree(c->jp);
   (void)hipFree(c->dm[0]);
   (void)hipFree(c->dm[1]);
-  (void)hipFree(c->luts);
+  if (c->luts_owned) (void)hipFree(c->luts);
   (void)hipFree(c->ctr);
   (void)hipFree(c->key);
   (void)hipFree(c->meas);
@@ -335,19 +463,36 @@ extern "C" int msa_destroy(msa_ctx *c) {
   return MSA_OK;
 }
 
-// canonical <-> colour-split index maps
-static inline uint64_t split_index(const Geo &g, uint64_t s, uint32_t *col) {
-  const uint32_t x = (uint32_t)(s % g.L);
-  const uint64_t yz = s / g.L;
-  const uint32_t y = (uint32_t)(yz % g.L), z = (uint32_t)(yz / g.L);
-  *col = (x + y + z) & 1u;
-  return (x >> 1) + (uint64_t)g.Lh * (y + (uint64_t)g.L * z);
+extern "C" int msa_set_tiling(msa_ctx *c, int32_t mode, int32_t ty, int32_t tz) {
+  if (!c) return msa_fail(MSA_ERR_ARG, "NULL context");
+  if (mode == 0) {
+    c->fused = 0;
+    return MSA_OK;
+  }
+  if (mode != 1) return msa_fail(MSA_ERR_ARG, "mode=%d (0 two-pass, 1 one-pass)", mode);
+  if (ty == 0 && tz == 0) {
+    pick_fused(c);
+    if (!c->fused) return msa_fail(MSA_ERR_ARG, "no one-pass tile for L=%u R=%u", c->g.L, c->g.R);
+    return MSA_OK;
+  }
+  if (!fused_tile_ok(c->g, (uint32_t)ty, (uint32_t)tz))
+    return msa_fail(MSA_ERR_ARG, "one-pass tile ty=%d tz=%d invalid for L=%u R=%u", ty, tz, c->g.L, c->g.R);
+  set_fused(c, (uint32_t)ty, (uint32_t)tz);
+  return MSA_OK;
+}
+
+extern "C" int msa_get_tiling(msa_ctx *c, int32_t *mode, int32_t *ty, int32_t *tz) {
+  if (!c || !mode || !ty || !tz) return msa_fail(MSA_ERR_ARG, "NULL argument");
+  *mode = c->fused;
+  *ty = (int32_t)c->fty;
+  *tz = (int32_t)c->ftz;
+  return MSA_OK;
 }
 
 extern "C" int msa_upload_state(msa_ctx *c, const uint64_t *J, const uint64_t *spin,
                                 const uint64_t *d0, const uint64_t *d1) {
-  if (!c || !J || !spin) return fail(MSA_ERR_ARG, "NULL argument");
-  if ((d0 == nullptr) != (d1 == nullptr)) return fail(MSA_ERR_ARG, "d0 and d1 must both be given");
+  if (!c || !J || !spin) return msa_fail(MSA_ERR_ARG, "NULL argument");
+  if ((d0 == nullptr) != (d1 == nullptr)) return msa_fail(MSA_ERR_ARG, "d0 and d1 must both be given");
   HIPCHK(hipSetDevice(c->device));
   const Geo &g = c->g;
   const uint64_t V = g.V, H = g.H;
@@ -358,8 +503,10 @@ extern "C" int msa_upload_state(msa_ctx *c, const uint64_t *J, const uint64_t *s
     for (int d = 0; d < 3; ++d) jb[(col * 3 + d) * H + h] = J[d * V + s];
     for (uint32_t r = 0; r < g.R; ++r) sb[(2 * r + col) * H + h] = spin[r * V + s];
   }
+  HIPCHK(hipStreamSynchronize(c->stream));
   HIPCHK(hipMemcpyAsync(c->jp, jb.data(), 8 * 3 * V, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(c->sp, sb.data(), 8 * g.R * V, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->sp[0], sb.data(), 8 * g.R * V, hipMemcpyHostToDevice, c->stream));
+  c->scur = 0;
   c->cur = 0;
   if (d0) {
     std::vector<ulonglong2> db((uint64_t)g.R * g.nc * V);
@@ -382,14 +529,14 @@ extern "C" int msa_upload_state(msa_ctx *c, const uint64_t *J, const uint64_t *s
 }
 
 extern "C" int msa_download_state(msa_ctx *c, uint64_t *spin, uint64_t *d0, uint64_t *d1) {
-  if (!c) return fail(MSA_ERR_ARG, "NULL context");
+  if (!c) return msa_fail(MSA_ERR_ARG, "NULL context");
   HIPCHK(hipSetDevice(c->device));
   const Geo &g = c->g;
   const uint64_t V = g.V, H = g.H;
   HIPCHK(hipStreamSynchronize(c->stream));
   if (spin) {
     std::vector<uint64_t> sb(g.R * V);
-    HIPCHK(hipMemcpy(sb.data(), c->sp, 8 * g.R * V, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(sb.data(), c->sp[c->scur], 8 * g.R * V, hipMemcpyDeviceToHost));
     for (uint64_t s = 0; s < V; ++s) {
       uint32_t col;
       const uint64_t h = split_index(g, s, &col);
@@ -415,7 +562,7 @@ extern "C" int msa_download_state(msa_ctx *c, uint64_t *spin, uint64_t *d0, uint
 }
 
 extern "C" int msa_download_couplings(msa_ctx *c, uint64_t *J) {
-  if (!c || !J) return fail(MSA_ERR_ARG, "NULL argument");
+  if (!c || !J) return msa_fail(MSA_ERR_ARG, "NULL argument");
   HIPCHK(hipSetDevice(c->device));
   const Geo &g = c->g;
   const uint64_t V = g.V, H = g.H;
@@ -431,57 +578,51 @@ extern "C" int msa_download_couplings(msa_ctx *c, uint64_t *J) {
 }
 
 extern "C" int msa_init_philox(msa_ctx *c, uint64_t seed) {
-  if (!c) return fail(MSA_ERR_ARG, "NULL context");
+  if (!c) return msa_fail(MSA_ERR_ARG, "NULL context");
   HIPCHK(hipSetDevice(c->device));
   const uint64_t V = c->g.V;
   const uint32_t nb = (uint32_t)((V + 255) / 256);
-  hipLaunchKernelGGL(k_init_philox, dim3(nb), dim3(256), 0, c->stream, c->sp, c->jp, c->g,
+  hipLaunchKernelGGL(k_init_philox, dim3(nb), dim3(256), 0, c->stream, c->sp[0], c->jp, c->g,
                      (uint32_t)seed, (uint32_t)(seed >> 32));
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemsetAsync(c->dm[0], 0, 16ull * c->g.R * c->g.nc * V, c->stream));
+  c->scur = 0;
   c->cur = 0;
   HIPCHK(hipStreamSynchronize(c->stream));
   return MSA_OK;
 }
 
-// A heat-bath table is usable when its thresholds are non-decreasing, the last
-// one is 2^64-1 and every presearch entry has min <= max <= 255.
-static bool lut_valid(const msa_lut &t) {
-  uint64_t prev = 0;
-  for (int n = 0; n < 256; ++n) {
-    const uint64_t u = ((uint64_t)t.umbrales[2 * n] << 32) | t.umbrales[2 * n + 1];
-    if (u < prev) return false;
-    prev = u;
-  }
-  if (prev != ~0ull) return false;
-  const uint16_t *pre = reinterpret_cast<const uint16_t *>(t.prebusqueda);
-  for (int m = 0; m < 1024; ++m)
-    if ((pre[m] & 255) > (pre[m] >> 8)) return false;
-  return true;
-}
-
 extern "C" int msa_set_luts(msa_ctx *c, int32_t nlut, const msa_lut *luts) {
-  if (!c || !luts || nlut < 1) return fail(MSA_ERR_ARG, "need at least one LUT pair");
+  if (!c || !luts || nlut < 1) return msa_fail(MSA_ERR_ARG, "need at least one LUT pair");
   for (int32_t k = 0; k < 2 * nlut; ++k)
-    if (!lut_valid(luts[k])) return fail(MSA_ERR_ARG, "LUT %d is malformed", k);
+    if (!lut_valid(luts[k])) return msa_fail(MSA_ERR_ARG, "LUT %d is malformed", k);
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipStreamSynchronize(c->stream));
-  (void)hipFree(c->luts);
+  if (c->luts_owned) (void)hipFree(c->luts);
   c->luts = nullptr;
+  c->luts_owned = true;
   c->nlut = 0;
   if (hipMalloc(&c->luts, sizeof(msa_lut) * 2 * (size_t)nlut) != hipSuccess)
-    return fail(MSA_ERR_NOMEM, "LUT table of %d pairs", nlut);
+    return msa_fail(MSA_ERR_NOMEM, "LUT table of %d pairs", nlut);
   HIPCHK(hipMemcpy(c->luts, luts, sizeof(msa_lut) * 2 * (size_t)nlut, hipMemcpyHostToDevice));
   c->nlut = nlut;
   return MSA_OK;
 }
 
+int msa_ctx_share_luts(msa_ctx *c, msa_lut *luts, int32_t nlut) {
+  if (c->luts_owned) (void)hipFree(c->luts);
+  c->luts = luts;
+  c->nlut = nlut;
+  c->luts_owned = false;
+  return MSA_OK;
+}
+
 extern "C" int msa_set_refresh(msa_ctx *c, int32_t nref, const msa_u32x4 *ctr,
                                const uint32_t *key, const int32_t *lut_idx) {
-  if (!c || !ctr || !key || !lut_idx || nref < 1) return fail(MSA_ERR_ARG, "bad refresh table");
+  if (!c || !ctr || !key || !lut_idx || nref < 1) return msa_fail(MSA_ERR_ARG, "bad refresh table");
   for (int32_t k = 0; k < nref; ++k)
     if (lut_idx[k] < 0 || lut_idx[k] >= c->nlut)
-      return fail(MSA_ERR_RANGE, "lut_idx[%d]=%d outside 0..%d (set LUTs first)", k, lut_idx[k],
+      return msa_fail(MSA_ERR_RANGE, "lut_idx[%d]=%d outside 0..%d (set LUTs first)", k, lut_idx[k],
                   c->nlut - 1);
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -493,7 +634,7 @@ extern "C" int msa_set_refresh(msa_ctx *c, int32_t nref, const msa_u32x4 *ctr,
   const size_t R = c->g.R;
   if (hipMalloc(&c->ctr, sizeof(uint4) * R * (size_t)nref) != hipSuccess ||
       hipMalloc(&c->key, sizeof(uint32_t) * R) != hipSuccess)
-    return fail(MSA_ERR_NOMEM, "refresh table of %d entries", nref);
+    return msa_fail(MSA_ERR_NOMEM, "refresh table of %d entries", nref);
   HIPCHK(hipMemcpy(c->ctr, ctr, sizeof(uint4) * R * (size_t)nref, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->key, key, sizeof(uint32_t) * R, hipMemcpyHostToDevice));
   c->lut_idx.assign(lut_idx, lut_idx + nref);
@@ -501,44 +642,70 @@ extern "C" int msa_set_refresh(msa_ctx *c, int32_t nref, const msa_u32x4 *ctr,
   return MSA_OK;
 }
 
-// refresh index of iteration t, -1 when t does not refresh
-static int64_t refresh_index(uint64_t t, int64_t every) {
-  if (every > 0) return (t % (uint64_t)every == 0) ? (int64_t)(t / (uint64_t)every) : -1;
-  if (every == -1) {
-    if (t == 0) return 0;
-    if (t & (t - 1)) return -1;
-    return 64 - __builtin_clzll(t);   // log2(t) + 1
+static void launch_fused(msa_ctx *c, const FusedArgs &fa) {
+  const dim3 grid(c->g.L / c->fty, c->g.L / c->ftz);
+  const dim3 block(c->g.Lh * c->fny);
+  const size_t lds = (size_t)3 * c->fny * c->g.Lh * c->g.R * 8;
+  switch (c->g.R) {
+    case 1: hipLaunchKernelGGL(k_sweep_fused<1>, grid, block, lds, c->stream, fa); break;
+    case 2: hipLaunchKernelGGL(k_sweep_fused<2>, grid, block, lds, c->stream, fa); break;
+    default: hipLaunchKernelGGL(k_sweep_fused<4>, grid, block, lds, c->stream, fa); break;
   }
-  return -1;
 }
 
 static int launch_run(msa_ctx *c, uint64_t t0, int64_t niter, int64_t every) {
-  if (niter < 0) return fail(MSA_ERR_ARG, "niter=%lld", (long long)niter);
-  if (every < -1) return fail(MSA_ERR_ARG, "refresh_every=%lld", (long long)every);
+  if (niter < 0) return msa_fail(MSA_ERR_ARG, "niter=%lld", (long long)niter);
+  if (every < -1) return msa_fail(MSA_ERR_ARG, "refresh_every=%lld", (long long)every);
   for (int64_t n = 0; n < niter; ++n) {
     const uint64_t t = t0 + (uint64_t)n;
     const int64_t k = refresh_index(t, every);
     if (k >= 0 && (k >= c->nref || c->nlut == 0))
-      return fail(MSA_ERR_RANGE, "iteration %llu needs refresh %lld but %d are set",
+      return msa_fail(MSA_ERR_RANGE, "iteration %llu needs refresh %lld but %d are set",
                   (unsigned long long)t, (long long)k, c->nref);
-    SweepArgs a{};
-    a.sp = c->sp;
-    a.jp = c->jp;
-    a.dm_in = c->dm[c->cur];
-    a.dm_out = c->dm[c->cur ^ 1];
-    a.g = c->g;
-    a.refresh = k >= 0;
-    if (k >= 0) {
-      a.ctr = c->ctr + (size_t)k * c->g.R;
-      a.key = c->key;
-      a.lut = c->luts + 2 * (size_t)c->lut_idx[k];
-    }
+    const ulonglong2 *dmi = c->dm[c->cur];
+    ulonglong2 *dmo = c->dm[c->cur ^ 1];
     for (uint32_t l = 0; l < c->g.nc; ++l) {
-      a.layer = (int)l;
-      for (int col = 0; col < 2; ++col) {
-        a.colour = col;
+      const uint64_t *A = c->sp[c->scur];
+      uint64_t *B = c->sp[c->scur ^ 1];
+      if (k < 0 && c->fused) {
+        FusedArgs fa{};
+        fa.sp_in = A;
+        fa.sp_out = B;
+        fa.jp = c->jp;
+        fa.dm_in = dmi;
+        fa.dm_out = dmo;
+        fa.g = c->g;
+        fa.layer = (int)l;
+        fa.ty = c->fty;
+        fa.tz = c->ftz;
+        fa.ny = c->fny;
+        fa.yhalo = c->fyhalo;
+        launch_fused(c, fa);
+      } else {
+        SweepArgs a{};
+        a.jp = c->jp;
+        a.dm_in = dmi;
+        a.dm_out = dmo;
+        a.g = c->g;
+        a.layer = (int)l;
+        a.refresh = k >= 0;
+        if (k >= 0) {
+          a.ctr = c->ctr + (size_t)k * c->g.R;
+          a.key = c->key;
+          a.lut = c->luts + 2 * (size_t)c->lut_idx[k];
+        }
+        a.colour = 0;
+        a.sp_own = A;
+        a.sp_nb = A;
+        a.sp_out = B;
+        hipLaunchKernelGGL(k_halfsweep, c->grd, c->blk, 0, c->stream, a);
+        a.colour = 1;
+        a.sp_own = A;
+        a.sp_nb = B;
+        a.sp_out = B;
         hipLaunchKernelGGL(k_halfsweep, c->grd, c->blk, 0, c->stream, a);
       }
+      c->scur ^= 1;
     }
     c->cur ^= 1;
   }
@@ -547,14 +714,14 @@ static int launch_run(msa_ctx *c, uint64_t t0, int64_t niter, int64_t every) {
 }
 
 extern "C" int msa_run(msa_ctx *c, uint64_t t0, int64_t niter, int64_t refresh_every) {
-  if (!c) return fail(MSA_ERR_ARG, "NULL context");
+  if (!c) return msa_fail(MSA_ERR_ARG, "NULL context");
   HIPCHK(hipSetDevice(c->device));
   return launch_run(c, t0, niter, refresh_every);
 }
 
 extern "C" int msa_run_timed(msa_ctx *c, uint64_t t0, int64_t niter, int64_t refresh_every,
                              float *ms) {
-  if (!c || !ms) return fail(MSA_ERR_ARG, "NULL argument");
+  if (!c || !ms) return msa_fail(MSA_ERR_ARG, "NULL argument");
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipEventRecord(c->ev0, c->stream));
   int rc = launch_run(c, t0, niter, refresh_every);
@@ -571,7 +738,7 @@ static int measure_one(msa_ctx *c, int what, int ra, int rb, int32_t *host_out) 
   const uint64_t nthr = (g.H + per_thread - 1) / per_thread;
   const uint32_t nb = (uint32_t)((nthr + 255) / 256);
   MeasArgs a{};
-  a.sp = c->sp;
+  a.sp = c->sp[c->scur];
   a.jp = c->jp;
   a.dm = c->dm[c->cur];
   a.g = g;
@@ -590,7 +757,7 @@ static int measure_one(msa_ctx *c, int what, int ra, int rb, int32_t *host_out) 
 
 extern "C" int msa_measure(msa_ctx *c, int32_t *unsat, int32_t *demon, int32_t *q,
                            int32_t *qlink) {
-  if (!c) return fail(MSA_ERR_ARG, "NULL context");
+  if (!c) return msa_fail(MSA_ERR_ARG, "NULL context");
   HIPCHK(hipSetDevice(c->device));
   int rc;
   for (uint32_t r = 0; r < c->g.R; ++r) {
@@ -604,8 +771,68 @@ extern "C" int msa_measure(msa_ctx *c, int32_t *unsat, int32_t *demon, int32_t *
   return MSA_OK;
 }
 
+// ---------------------------------------------------------- autotuner ----
+// Choice per (L, R, nc), measured once per process and reused.
+struct TuneEntry {
+  uint32_t L, R, nc;
+  int32_t mode, ty, tz;
+  float ms;
+};
+static std::vector<TuneEntry> g_tuned;
+
+extern "C" int msa_autotune(msa_ctx *c, int32_t sweeps, int32_t *mode, int32_t *ty, int32_t *tz,
+                            float *ms_per_sweep) {
+  if (!c) return msa_fail(MSA_ERR_ARG, "NULL context");
+  if (sweeps < 1) sweeps = 8;
+  for (const TuneEntry &e : g_tuned)
+    if (e.L == c->g.L && e.R == c->g.R && e.nc == c->g.nc) {
+      int rc = msa_set_tiling(c, e.mode, e.ty, e.tz);
+      if (rc) return rc;
+      if (mode) *mode = e.mode;
+      if (ty) *ty = e.ty;
+      if (tz) *tz = e.tz;
+      if (ms_per_sweep) *ms_per_sweep = e.ms;
+      return MSA_OK;
+    }
+  // candidates: the two-pass form and every valid one-pass tile with slabs of
+  // at most 32 planes
+  struct Cand { int32_t mode, ty, tz; };
+  std::vector<Cand> cands{{0, 0, 0}};
+  for (uint32_t y = 1; y <= c->g.L; ++y)
+    for (uint32_t z = 2; z <= c->g.L && z <= 32; ++z)
+      if (fused_tile_ok(c->g, y, z)) cands.push_back({1, (int32_t)y, (int32_t)z});
+  msa_ctx *t = nullptr;
+  int rc = msa_create(c->device, &c->geom, &t);
+  if (rc) return rc;
+  rc = msa_init_philox(t, 0xA070u);
+  TuneEntry best{c->g.L, c->g.R, c->g.nc, 0, 0, 0, 1e30f};
+  for (const Cand &cd : cands) {
+    if (rc) break;
+    if ((rc = msa_set_tiling(t, cd.mode, cd.ty, cd.tz))) break;
+    float ms = 0;
+    if ((rc = msa_run(t, 0, 2, 0))) break;                      // warm-up
+    if ((rc = msa_run_timed(t, 0, sweeps, 0, &ms))) break;
+    const float per = ms / (float)sweeps;
+    if (per < best.ms) {
+      best.mode = cd.mode;
+      best.ty = cd.ty;
+      best.tz = cd.tz;
+      best.ms = per;
+    }
+  }
+  msa_destroy(t);
+  if (rc) return rc;
+  g_tuned.push_back(best);
+  if ((rc = msa_set_tiling(c, best.mode, best.ty, best.tz))) return rc;
+  if (mode) *mode = best.mode;
+  if (ty) *ty = best.ty;
+  if (tz) *tz = best.tz;
+  if (ms_per_sweep) *ms_per_sweep = best.ms;
+  return MSA_OK;
+}
+
 extern "C" int msa_sync(msa_ctx *c) {
-  if (!c) return fail(MSA_ERR_ARG, "NULL context");
+  if (!c) return msa_fail(MSA_ERR_ARG, "NULL context");
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipStreamSynchronize(c->stream));
   return MSA_OK;

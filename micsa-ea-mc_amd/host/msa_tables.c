@@ -6,6 +6,7 @@
  * demon refresh can draw its two bit planes from it.  Thresholds:
  * get_thresholds, PROGRAM/src/ini.c:680-710.
  */
+#include <math.h>
 #include <quadmath.h>
 #include <string.h>
 
@@ -115,5 +116,34 @@ int msa_sched_tables(const msa_sched *s, int64_t nref, int R, const uint64_t *se
   }
 #pragma omp parallel for schedule(dynamic, 16)
   for (int64_t k = 0; k < nref; k++) msa_lut_demon(msa_sched_beta(s, k), &luts[2 * k], &luts[2 * k + 1]);
+  return 0;
+}
+
+/* ------------------------------------------------- shared grid tables -- */
+
+void msa_lut_grid(double tgrid, int32_t nlo, int32_t nlut, msa_lut *luts)
+{
+#pragma omp parallel for schedule(dynamic, 8)
+  for (int32_t n = 0; n < nlut; n++)
+    msa_lut_demon(1.0 / (tgrid * (double)(nlo + n)), &luts[2 * n], &luts[2 * n + 1]);
+}
+
+int msa_sched_tables_grid(const msa_sched *s, int64_t nref, int R, const uint64_t *seed_MC,
+                          uint32_t jobid, double tgrid, int32_t nlo, int32_t nlut,
+                          msa_u32x4 *ctr, uint32_t *keys, int32_t *lut_idx)
+{
+  if (R < 1 || R > MSA_MAXR || nref < 0 || tgrid <= 0 || nlut < 1) return -1;
+  msa_prng pr[MSA_MAXR];
+  msa_xo256 xo[MSA_MAXR];
+  for (int r = 0; r < R; r++) msa_cpu_generators(&pr[r], &xo[r], &keys[r], seed_MC[r]);
+  for (int64_t k = 0; k < nref; k++) {
+    const uint64_t t = (uint64_t)msa_sched_time(s, k);
+    for (int r = 0; r < R; r++) msa_fill_counter(&pr[r], &xo[r], t, 0, jobid, &ctr[k * R + r]);
+    const double T = 1.0 / msa_sched_beta(s, k);
+    int64_t n = (int64_t)floor(T / tgrid + 0.5) - nlo;
+    if (n < 0) n = 0;
+    if (n >= nlut) n = nlut - 1;
+    lut_idx[k] = (int32_t)n;
+  }
   return 0;
 }

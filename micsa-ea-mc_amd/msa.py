@@ -29,6 +29,12 @@ class Geom(C.Structure):
 LUT_BYTES = 4096        # one msa_lut / s_lut_heat_bath
 
 
+class Job(C.Structure):
+    _fields_ = [("geom", Geom), ("seed", C.c_uint64), ("niter", C.c_int64),
+                ("refresh_every", C.c_int64), ("nref", C.c_int32), ("ctr", u32p),
+                ("key", u32p), ("lut_idx", i32p)]
+
+
 def _p(a, t=u64p):
     return None if a is None else a.ctypes.data_as(t)
 
@@ -62,6 +68,26 @@ def hip():
         L.msa_run_timed.argtypes = [vp, C.c_uint64, C.c_int64, C.c_int64, C.POINTER(C.c_float)]
         L.msa_measure.argtypes = [vp, i32p, i32p, i32p, i32p]
         L.msa_sync.argtypes = [vp]
+        L.msa_set_tiling.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32]
+        L.msa_get_tiling.argtypes = [vp, i32p, i32p, i32p]
+        bp = C.c_void_p
+        L.msa_batch_create.argtypes = [C.c_int, C.c_int64, C.POINTER(bp)]
+        L.msa_batch_destroy.argtypes = [bp]
+        L.msa_batch_set_luts.argtypes = [bp, C.c_int32, C.c_void_p]
+        L.msa_batch_add.argtypes = [bp, C.POINTER(Job), i32p]
+        L.msa_batch_job_bytes.argtypes = [C.POINTER(Job)]
+        L.msa_batch_job_bytes.restype = C.c_int64
+        L.msa_batch_bytes_in_use.argtypes = [bp]
+        L.msa_batch_bytes_in_use.restype = C.c_int64
+        L.msa_batch_upload_state.argtypes = [bp, C.c_int32, u64p, u64p, u64p, u64p]
+        L.msa_batch_set_shape.argtypes = [bp, C.c_int32, C.c_int32]
+        L.msa_batch_run.argtypes = [bp, C.c_int64]
+        L.msa_batch_sync.argtypes = [bp]
+        L.msa_batch_progress.argtypes = [bp, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        L.msa_batch_measure.argtypes = [bp, C.c_int32, i32p, i32p, i32p, i32p]
+        L.msa_batch_download_state.argtypes = [bp, C.c_int32, u64p, u64p, u64p]
+        L.msa_batch_remove.argtypes = [bp, C.c_int32]
+        L.msa_autotune.argtypes = [vp, C.c_int32, i32p, i32p, i32p, C.POINTER(C.c_float)]
         L.msa_footprint_bytes.argtypes = [C.POINTER(Geom)]
         L.msa_footprint_bytes.restype = C.c_int64
         L.msa_strerror.argtypes = [C.c_int]
@@ -147,6 +173,20 @@ class Lattice:
         _check(self.lib.msa_run_timed(self.ctx, t0, niter, refresh_every, C.byref(ms)))
         return ms.value
 
+    def set_tiling(self, mode, ty=0, tz=0):
+        _check(self.lib.msa_set_tiling(self.ctx, mode, ty, tz))
+
+    def autotune(self, sweeps=8):
+        m, ty, tz, ms = C.c_int32(), C.c_int32(), C.c_int32(), C.c_float()
+        _check(self.lib.msa_autotune(self.ctx, sweeps, C.byref(m), C.byref(ty), C.byref(tz),
+                                     C.byref(ms)))
+        return m.value, ty.value, tz.value, ms.value
+
+    def tiling(self):
+        m, ty, tz = C.c_int32(), C.c_int32(), C.c_int32()
+        _check(self.lib.msa_get_tiling(self.ctx, C.byref(m), C.byref(ty), C.byref(tz)))
+        return m.value, ty.value, tz.value
+
     def sync(self):
         _check(self.lib.msa_sync(self.ctx))
 
@@ -217,8 +257,33 @@ def host():
         L.msa_sched_beta.restype = C.c_double
         L.msa_sched_tables.argtypes = [C.POINTER(Sched), C.c_int64, C.c_int, u64p, C.c_uint32,
                                        u32p, u32p, C.c_void_p, i32p]
+        L.msa_lut_grid.argtypes = [C.c_double, C.c_int32, C.c_int32, C.c_void_p]
+        L.msa_sched_tables_grid.argtypes = [C.POINTER(Sched), C.c_int64, C.c_int, u64p, C.c_uint32,
+                                            C.c_double, C.c_int32, C.c_int32, u32p, u32p, i32p]
         _host = L
     return _host
+
+
+def lut_grid(tgrid, nlo, nlut):
+    """Shared LUT pairs for temperatures tgrid*(nlo .. nlo+nlut-1), as bytes."""
+    out = np.zeros(2 * nlut * LUT_BYTES, np.uint8)
+    host().msa_lut_grid(tgrid, nlo, nlut, out.ctypes.data_as(C.c_void_p))
+    return out
+
+
+def schedule_tables_grid(sched, niter, R, seed_mc, jobid, tgrid, nlo, nlut):
+    """Counters (nref,R,4), keys (R,), indices into the shared grid table (nref,)."""
+    H = host()
+    nref = H.msa_sched_count(C.byref(sched), niter)
+    ctr = np.zeros((max(nref, 1), R, 4), np.uint32)
+    keys = np.zeros(R, np.uint32)
+    idx = np.zeros(max(nref, 1), np.int32)
+    seeds = np.ascontiguousarray(seed_mc, np.uint64)
+    rc = H.msa_sched_tables_grid(C.byref(sched), nref, R, _p(seeds), jobid, tgrid, nlo, nlut,
+                                 _p(ctr, u32p), _p(keys, u32p), _p(idx, i32p))
+    if rc:
+        raise MsaError("grid schedule tables failed")
+    return ctr, keys, idx
 
 
 def lut_demon(beta):
@@ -263,3 +328,98 @@ def schedule_tables(sched, niter, R, seed_mc, jobid=0):
     if rc:
         raise MsaError("schedule tables failed")
     return ctr, keys, luts, idx
+
+
+class Batch:
+    """Many lattices advanced together (msa_batch_* of include/msa.h)."""
+
+    def __init__(self, luts, max_bytes=0, device=0):
+        self.lib = hip()
+        self.b = C.c_void_p()
+        _check(self.lib.msa_batch_create(device, max_bytes, C.byref(self.b)))
+        raw = np.frombuffer(bytes(luts), np.uint8) if not isinstance(luts, np.ndarray) else luts
+        self._luts = np.ascontiguousarray(raw, np.uint8)
+        _check(self.lib.msa_batch_set_luts(self.b, self._luts.size // (2 * LUT_BYTES),
+                                           self._luts.ctypes.data_as(C.c_void_p)))
+        self.geom = {}
+
+    def close(self):
+        if self.b:
+            self.lib.msa_batch_destroy(self.b)
+            self.b = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def make_job(L, R, seed, niter, refresh_every, ctr, key, lut_idx):
+        ctr = np.ascontiguousarray(ctr, np.uint32).reshape(-1, R, 4)
+        key = np.ascontiguousarray(key, np.uint32)
+        lut_idx = np.ascontiguousarray(lut_idx, np.int32)
+        j = Job(Geom(L, R, 1), seed, niter, refresh_every, ctr.shape[0], _p(ctr, u32p),
+                _p(key, u32p), _p(lut_idx, i32p))
+        j._keep = (ctr, key, lut_idx)
+        return j
+
+    def job_bytes(self, job):
+        return self.lib.msa_batch_job_bytes(C.byref(job))
+
+    def add(self, job):
+        """Slot of the admitted lattice, or None when it does not fit (admission by footprint)."""
+        slot = C.c_int32()
+        rc = self.lib.msa_batch_add(self.b, C.byref(job), C.byref(slot))
+        if rc == -3:
+            return None
+        _check(rc)
+        self.geom[slot.value] = (job.geom.L, job.geom.R)
+        return slot.value
+
+    def bytes_in_use(self):
+        return self.lib.msa_batch_bytes_in_use(self.b)
+
+    def upload(self, slot, J, S, D0=None, D1=None):
+        J = np.ascontiguousarray(J, np.uint64)
+        S = np.ascontiguousarray(S, np.uint64)
+        if D0 is not None:
+            D0 = np.ascontiguousarray(D0, np.uint64)
+            D1 = np.ascontiguousarray(D1, np.uint64)
+        _check(self.lib.msa_batch_upload_state(self.b, slot, _p(J), _p(S), _p(D0), _p(D1)))
+
+    def set_shape(self, block, chunk):
+        _check(self.lib.msa_batch_set_shape(self.b, block, chunk))
+
+    def run(self, niter):
+        _check(self.lib.msa_batch_run(self.b, niter))
+
+    def sync(self):
+        _check(self.lib.msa_batch_sync(self.b))
+
+    def progress(self, nslots):
+        t = np.zeros(nslots, np.int64)
+        te = np.zeros(nslots, np.int64)
+        _check(self.lib.msa_batch_progress(self.b, nslots, t.ctypes.data_as(C.POINTER(C.c_int64)),
+                                           te.ctypes.data_as(C.POINTER(C.c_int64))))
+        return t, te
+
+    def download(self, slot):
+        L, R = self.geom[slot]
+        V = L ** 3
+        S = np.zeros((R, V), np.uint64)
+        D0 = np.zeros((R, 1, V), np.uint64)
+        D1 = np.zeros_like(D0)
+        _check(self.lib.msa_batch_download_state(self.b, slot, _p(S), _p(D0), _p(D1)))
+        return S, D0, D1
+
+    def measure(self, slot):
+        L, R = self.geom[slot]
+        unsat = np.zeros((R, 64), np.int32)
+        demon = np.zeros((R, 64), np.int32)
+        _check(self.lib.msa_batch_measure(self.b, slot, _p(unsat, i32p), _p(demon, i32p), None, None))
+        return {"unsat": unsat, "energy": 3 * L ** 3 - 2 * unsat, "demon": demon}
+
+    def remove(self, slot):
+        _check(self.lib.msa_batch_remove(self.b, slot))
+        self.geom.pop(slot, None)

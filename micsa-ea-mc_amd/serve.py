@@ -1,0 +1,135 @@
+"""Served form: a request loop with continuous batching on one GPU.
+
+Requests (annealing runs of different L and lengths) arrive over time.  A
+request is admitted into the running batch (include/msa.h msa_batch_*) as
+soon as it has arrived and its device footprint fits under the memory limit;
+admitted lattices join the next launches while the others keep running, and
+finished ones are retired, freeing their memory for the queue.  All state
+(lattice, demons, counters, tables) stays on the device from admission to
+retirement.  Every result is bit-identical to the same request run alone,
+because each lattice's update depends only on its own state and tables.
+
+No reference counterpart: the reference runs one sample set per process.
+"""
+import dataclasses
+import time
+
+import numpy as np
+
+from . import msa
+
+TGRID = 1.0 / 1024          # temperature grid of the shared LUT table
+T_LO, T_HI = 0.5, 2.5
+
+
+@dataclasses.dataclass
+class Request:
+    rid: int
+    L: int
+    niter: int
+    T0: float
+    T1: float
+    seed: int
+    arrival: float          # seconds after the start of the stream
+    R: int = 1
+
+
+def make_requests(n, rate, seed=5151, sizes=(16, 32, 48, 64, 96, 128), log2_len=(12, 17),
+                  T0=2.0, T1=(0.6, 0.9)):
+    """Poisson stream: exponential gaps of mean 1/rate, L uniform over `sizes`,
+    length 2^U{log2_len}, final temperature uniform in T1, seed = request id."""
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, size=n)
+    arr = np.cumsum(gaps) - gaps[0]
+    out = []
+    for i in range(n):
+        L = int(rng.choice(sizes))
+        niter = 1 << int(rng.integers(log2_len[0], log2_len[1] + 1))
+        t1 = float(rng.uniform(*T1))
+        out.append(Request(i, L, niter, T0, t1, 1000003 * (i + 1), float(arr[i])))
+    return out
+
+
+class Tables:
+    """The shared temperature-grid LUT table and per-request refresh tables."""
+
+    def __init__(self):
+        self.nlo = int(round(T_LO / TGRID))
+        self.nlut = int(round(T_HI / TGRID)) - self.nlo + 1
+        self.luts = msa.lut_grid(TGRID, self.nlo, self.nlut)
+
+    def job(self, req, jobid):
+        sched = msa.schedule(req.T0, req.T1, req.niter, 1)
+        ctr, keys, idx = msa.schedule_tables_grid(sched, req.niter, req.R,
+                                                  [req.seed + r for r in range(req.R)],
+                                                  jobid % 16384, TGRID, self.nlo, self.nlut)
+        return msa.Batch.make_job(req.L, req.R, req.seed, req.niter, 1, ctr, keys, idx)
+
+
+def run_alone(req, tables, jobid):
+    """The request by itself (a batch holding only it): the reference result."""
+    b = msa.Batch(tables.luts)
+    slot = b.add(tables.job(req, jobid))
+    b.run(req.niter)
+    b.sync()
+    out = b.download(slot), b.measure(slot)
+    b.close()
+    return out
+
+
+def serve(requests, max_bytes, epoch=64, block=256, tables=None, keep_state=False, clock=None):
+    """Run the stream; returns (per-request dicts, summary dict)."""
+    tables = tables or Tables()
+    batch = msa.Batch(tables.luts, max_bytes=max_bytes)
+    batch.set_shape(block, epoch)
+    reqs = sorted(requests, key=lambda r: r.arrival)
+    clock = clock or time.perf_counter
+    t_start = clock()
+    nxt = 0
+    queue = []
+    active = {}                 # slot -> (request, admit time)
+    done = {}
+    updates = 0
+    max_active = 0
+    while len(done) < len(reqs):
+        now = clock() - t_start
+        while nxt < len(reqs) and reqs[nxt].arrival <= now:
+            queue.append(reqs[nxt])
+            nxt += 1
+        while queue:                              # FIFO admission by footprint
+            req = queue[0]
+            slot = batch.add(tables.job(req, req.rid))
+            if slot is None:
+                if not active:
+                    raise RuntimeError("request %d alone exceeds the memory limit" % req.rid)
+                break
+            queue.pop(0)
+            active[slot] = (req, clock() - t_start)
+        max_active = max(max_active, len(active))
+        if not active:
+            if nxt < len(reqs):
+                time.sleep(max(0.0, min(0.01, reqs[nxt].arrival - (clock() - t_start))))
+            continue
+        batch.run(epoch)
+        nslots = max(active) + 1
+        t, tend = batch.progress(nslots)
+        for slot in [s for s in active if t[s] >= tend[s]]:
+            req, t_admit = active.pop(slot)
+            fin = clock() - t_start
+            rec = {"rid": req.rid, "L": req.L, "niter": req.niter, "arrival": req.arrival,
+                   "admit": t_admit, "finish": fin, "latency": fin - req.arrival,
+                   "energy": batch.measure(slot)["energy"]}
+            if keep_state:
+                rec["state"] = batch.download(slot)
+            done[req.rid] = rec
+            updates += 64 * req.R * req.L ** 3 * req.niter
+            batch.remove(slot)
+    makespan = clock() - t_start
+    batch.close()
+    lat = np.array([done[r.rid]["latency"] for r in reqs])
+    summary = {"requests": len(reqs), "makespan_s": makespan,
+               "spin_updates_per_ps": updates / makespan / 1e12,
+               "latency_p50_s": float(np.percentile(lat, 50)),
+               "latency_p99_s": float(np.percentile(lat, 99)),
+               "max_concurrent": max_active, "epoch": epoch}
+    return done, summary

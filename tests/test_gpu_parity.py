@@ -123,3 +123,52 @@ def test_errors_are_reported(pkg):
     with pytest.raises(pkg.MsaError):
         dev.run(0, 1, 1)          # refresh requested, no tables uploaded
     dev.close()
+
+
+@pytest.mark.parametrize("L,R,nc,ty,tz", [
+    (16, 1, 1, 16, 16),     # whole planes, whole lattice: halos wrap onto the block itself
+    (16, 2, 1, 4, 4),       # row and plane halos
+    (24, 2, 3, 6, 8),       # three layers: walk along x, y, z
+    (32, 4, 1, 8, 16),
+    (20, 2, 6, 5, 10),      # all six walk directions
+    (40, 1, 2, 40, 8),
+])
+def test_one_pass_sweep_matches_oracle(pkg, orc, L, R, nc, ty, tz):
+    print("case L=%d R=%d nc=%d tile %dx%d" % (L, R, nc, ty, tz), flush=True)
+    o = random_state(L, R, nc, 4242 + L)
+    every, niter = 3, 7
+    times = list(range(0, niter, every))
+    ctr, keys, raw, luts = oracle_schedule(orc, R, times, [0.8] * len(times), jobid=1)
+    dev = pkg.Lattice(L, R, nc)
+    dev.set_tiling(1, ty, tz)
+    assert dev.tiling() == (1, ty, tz)
+    dev.upload(o.J, o.S, o.D0, o.D1)
+    dev.set_luts(raw)
+    dev.set_refresh(ctr, keys, np.arange(len(times), dtype=np.int32))
+    for t in range(niter):
+        dev.run(t, 1, every)
+        if t % every == 0:
+            orc.iteration(o, ctr[t // every], keys, luts[t // every])
+        else:
+            orc.iteration(o)
+        assert_same_state(dev, o, "t=%d" % t)
+    assert_same_measures(dev, orc, o)
+    dev.close()
+
+
+def test_default_tiling_matches_two_pass(pkg):
+    L, R = 32, 2
+    a = pkg.Lattice(L, R, 1)
+    b = pkg.Lattice(L, R, 1)
+    assert a.tiling()[0] == 1
+    b.set_tiling(0)
+    for d in (a, b):
+        d.init_philox(99)
+        d.upload(d.couplings(), d.download()[0],
+                 *[np.random.default_rng(3).integers(0, 2 ** 64, size=(R, 1, L ** 3), dtype=np.uint64)
+                   for _ in range(2)])
+        d.run(0, 9, 0)
+    for x, y in zip(a.download(), b.download()):
+        assert np.array_equal(x, y)
+    a.close()
+    b.close()
