@@ -186,6 +186,12 @@ int msa_batch_download_state(msa_batch *b, int32_t slot, uint64_t *spin, uint64_
 /* Retire a lattice and free its memory. */
 int msa_batch_remove(msa_batch *b, int32_t slot);
 
+/* HBM copy probe (no reference counterpart): copies `bytes` between two
+ * fresh device buffers `iters` times with a plain streaming kernel and
+ * reports the mean time per copy; calibrates bandwidth counters and gives
+ * the practical bandwidth ceiling (2*bytes moved per copy). */
+int msa_copy_probe(int device, int64_t bytes, int32_t iters, float *ms_per_copy);
+
 /* Wait for all work on the context stream. */
 int msa_sync(msa_ctx *ctx);
 

@@ -127,6 +127,7 @@ struct msa_batch {
   std::vector<int64_t> slot_bytes;
   std::vector<int32_t *> slot_lut_idx;  // device copies
   std::vector<BLat> h_lats;
+  std::vector<char> fresh;              // added since the device table was last pushed
   BLat *d_lats = nullptr;
   size_t cap = 0;
   BItem *d_items = nullptr;
@@ -145,6 +146,21 @@ static int batch_clear_graph(msa_batch *b) {
   if (b->graph) (void)hipGraphDestroy(b->graph);
   b->gexec = nullptr;
   b->graph = nullptr;
+  return MSA_OK;
+}
+
+// Bring the host copy of the iteration counters up to date from the device
+// table for every lattice that is in that table (not the ones added since it
+// was pushed; a slot can be reused, and its new lattice may even get the old
+// one's buffers back from the allocator, so identity is tracked explicitly).
+static int batch_pull(msa_batch *b) {
+  if (!b->d_lats || !b->cap) return MSA_OK;
+  HIPCHK(hipStreamSynchronize(b->stream));
+  std::vector<BLat> cur(b->cap);
+  HIPCHK(hipMemcpyAsync(cur.data(), b->d_lats, sizeof(BLat) * b->cap, hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  for (size_t s = 0; s < std::min(b->slot.size(), b->cap); ++s)
+    if (b->slot[s] && !b->fresh[s]) b->h_lats[s].t = cur[s].t;
   return MSA_OK;
 }
 
@@ -194,7 +210,9 @@ extern "C" int msa_batch_set_luts(msa_batch *b, int32_t nlut, const msa_lut *lut
   b->luts = nullptr;
   if (hipMalloc(&b->luts, sizeof(msa_lut) * 2 * (size_t)nlut) != hipSuccess)
     return msa_fail(MSA_ERR_NOMEM, "LUT table of %d pairs", nlut);
-  HIPCHK(hipMemcpy(b->luts, luts, sizeof(msa_lut) * 2 * (size_t)nlut, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpyAsync(b->luts, luts, sizeof(msa_lut) * 2 * (size_t)nlut, hipMemcpyHostToDevice,
+                        b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
   b->nlut = nlut;
   return MSA_OK;
 }
@@ -241,11 +259,18 @@ extern "C" int msa_batch_add(msa_batch *b, const msa_job *j, int32_t *slot_out) 
       msa_destroy(c);
       return msa_fail(MSA_ERR_NOMEM, "refresh index table");
     }
-    if (hipMemcpy(d_idx, j->lut_idx, sizeof(int32_t) * (size_t)j->nref, hipMemcpyHostToDevice) != hipSuccess) {
+    if (hipMemcpyAsync(d_idx, j->lut_idx, sizeof(int32_t) * (size_t)j->nref, hipMemcpyHostToDevice,
+                       b->stream) != hipSuccess ||
+        hipStreamSynchronize(b->stream) != hipSuccess) {
       (void)hipFree(d_idx);
       msa_destroy(c);
       return msa_fail(MSA_ERR_HIP, "refresh index upload");
     }
+  }
+  if ((rc = batch_pull(b))) {
+    (void)hipFree(d_idx);
+    msa_destroy(c);
+    return rc;
   }
   size_t s = 0;
   while (s < b->slot.size() && b->slot[s]) ++s;
@@ -254,7 +279,9 @@ extern "C" int msa_batch_add(msa_batch *b, const msa_job *j, int32_t *slot_out) 
     b->slot_bytes.push_back(0);
     b->slot_lut_idx.push_back(nullptr);
     b->h_lats.push_back(BLat{});
+    b->fresh.push_back(0);
   }
+  b->fresh[s] = 1;
   b->slot[s] = c;
   b->slot_bytes[s] = bytes;
   b->slot_lut_idx[s] = d_idx;
@@ -286,9 +313,10 @@ extern "C" int msa_batch_upload_state(msa_batch *b, int32_t slot, const uint64_t
     return msa_fail(MSA_ERR_ARG, "no lattice in slot %d", slot);
   HIPCHK(hipSetDevice(b->device));
   HIPCHK(hipStreamSynchronize(b->stream));
-  uint64_t t = 0;
-  HIPCHK(hipMemcpy(&t, &b->d_lats[slot].t, sizeof t, hipMemcpyDeviceToHost));
-  if (!b->dirty && t != 0) return msa_fail(MSA_ERR_STATE, "slot %d already ran %llu iterations", slot, (unsigned long long)t);
+  int rc = batch_pull(b);
+  if (rc) return rc;
+  const uint64_t t = b->h_lats[slot].t;
+  if (t != 0) return msa_fail(MSA_ERR_STATE, "slot %d already ran %llu iterations", slot, (unsigned long long)t);
   return msa_upload_state(b->slot[slot], J, spin, d0, d1);
 }
 
@@ -296,13 +324,8 @@ extern "C" int msa_batch_upload_state(msa_batch *b, int32_t slot, const uint64_t
 static int batch_prepare(msa_batch *b) {
   if (!b->dirty) return MSA_OK;
   const size_t n = b->slot.size();
-  // keep device-side progress of existing lattices
-  if (b->d_lats && b->cap) {
-    std::vector<BLat> cur(b->cap);
-    HIPCHK(hipMemcpy(cur.data(), b->d_lats, sizeof(BLat) * b->cap, hipMemcpyDeviceToHost));
-    for (size_t s = 0; s < std::min(n, b->cap); ++s)
-      if (b->slot[s] && cur[s].jp == b->h_lats[s].jp) b->h_lats[s].t = cur[s].t;
-  }
+  int rc = batch_pull(b);               // keep the progress of lattices already running
+  if (rc) return rc;
   std::vector<BItem> items;
   for (size_t s = 0; s < n; ++s) {
     if (!b->slot[s]) continue;
@@ -319,7 +342,8 @@ static int batch_prepare(msa_batch *b) {
     if (hipMalloc(&b->d_lats, sizeof(BLat) * n) != hipSuccess) return msa_fail(MSA_ERR_NOMEM, "descriptors");
     b->cap = n;
   }
-  if (n) HIPCHK(hipMemcpy(b->d_lats, b->h_lats.data(), sizeof(BLat) * n, hipMemcpyHostToDevice));
+  if (n) HIPCHK(hipMemcpyAsync(b->d_lats, b->h_lats.data(), sizeof(BLat) * n, hipMemcpyHostToDevice, b->stream));
+  for (size_t s = 0; s < n; ++s) b->fresh[s] = 0;
   if (items.size() > b->items_cap) {
     (void)hipFree(b->d_items);
     b->d_items = nullptr;
@@ -329,7 +353,9 @@ static int batch_prepare(msa_batch *b) {
     b->items_cap = (uint32_t)items.size();
   }
   if (!items.empty())
-    HIPCHK(hipMemcpy(b->d_items, items.data(), sizeof(BItem) * items.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpyAsync(b->d_items, items.data(), sizeof(BItem) * items.size(), hipMemcpyHostToDevice,
+                          b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
   b->nitems = (uint32_t)items.size();
   batch_clear_graph(b);
   if (b->nitems) {
@@ -403,14 +429,11 @@ extern "C" int msa_batch_sync(msa_batch *b) {
 extern "C" int msa_batch_progress(msa_batch *b, int32_t nslots, int64_t *t, int64_t *tend) {
   if (!b || nslots < 0) return msa_fail(MSA_ERR_ARG, "bad argument");
   HIPCHK(hipSetDevice(b->device));
-  HIPCHK(hipStreamSynchronize(b->stream));
-  std::vector<BLat> cur(b->cap);
-  if (b->cap && !b->dirty)
-    HIPCHK(hipMemcpy(cur.data(), b->d_lats, sizeof(BLat) * b->cap, hipMemcpyDeviceToHost));
+  int rc = batch_pull(b);
+  if (rc) return rc;
   for (int32_t s = 0; s < nslots; ++s) {
     const bool live = (size_t)s < b->slot.size() && b->slot[s];
-    const uint64_t ts = !live ? 0 : (b->dirty || (size_t)s >= b->cap) ? b->h_lats[s].t : cur[s].t;
-    if (t) t[s] = live ? (int64_t)ts : -1;
+    if (t) t[s] = live ? (int64_t)b->h_lats[s].t : -1;
     if (tend) tend[s] = live ? (int64_t)b->h_lats[s].tend : -1;
   }
   return MSA_OK;
@@ -421,10 +444,9 @@ static int batch_ctx(msa_batch *b, int32_t slot, msa_ctx **out) {
   if (!b || slot < 0 || (size_t)slot >= b->slot.size() || !b->slot[slot])
     return msa_fail(MSA_ERR_ARG, "no lattice in slot %d", slot);
   HIPCHK(hipSetDevice(b->device));
-  HIPCHK(hipStreamSynchronize(b->stream));
-  uint64_t t = b->h_lats[slot].t;
-  if (!b->dirty && (size_t)slot < b->cap)
-    HIPCHK(hipMemcpy(&t, &b->d_lats[slot].t, sizeof t, hipMemcpyDeviceToHost));
+  int rc = batch_pull(b);
+  if (rc) return rc;
+  const uint64_t t = b->h_lats[slot].t;
   msa_ctx *c = b->slot[slot];
   c->scur = c->cur = (int)(t & 1u);
   *out = c;
@@ -449,14 +471,8 @@ extern "C" int msa_batch_remove(msa_batch *b, int32_t slot) {
   if (!b || slot < 0 || (size_t)slot >= b->slot.size() || !b->slot[slot])
     return msa_fail(MSA_ERR_ARG, "no lattice in slot %d", slot);
   HIPCHK(hipSetDevice(b->device));
-  HIPCHK(hipStreamSynchronize(b->stream));
-  // keep the progress of the others before the descriptor table is rebuilt
-  if (!b->dirty && b->cap) {
-    std::vector<BLat> cur(b->cap);
-    HIPCHK(hipMemcpy(cur.data(), b->d_lats, sizeof(BLat) * b->cap, hipMemcpyDeviceToHost));
-    for (size_t s = 0; s < std::min(b->slot.size(), b->cap); ++s)
-      if (b->slot[s]) b->h_lats[s].t = cur[s].t;
-  }
+  int rc = batch_pull(b);               // keep the progress of the others
+  if (rc) return rc;
   msa_destroy(b->slot[slot]);
   (void)hipFree(b->slot_lut_idx[slot]);
   b->slot[slot] = nullptr;
@@ -464,6 +480,7 @@ extern "C" int msa_batch_remove(msa_batch *b, int32_t slot) {
   b->used_bytes -= b->slot_bytes[slot];
   b->slot_bytes[slot] = 0;
   b->h_lats[slot] = BLat{};
+  b->fresh[slot] = 0;
   b->dirty = true;
   return MSA_OK;
 }

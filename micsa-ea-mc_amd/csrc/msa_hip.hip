@@ -433,9 +433,12 @@ extern "C" int msa_create(int device, const msa_geom *geom, msa_ctx **out) {
       hipMalloc(&c->meas, 64 * sizeof(int32_t)) != hipSuccess)
     return cleanup(msa_fail(MSA_ERR_NOMEM, "device allocation of %lld bytes failed",
                         (long long)msa_footprint_bytes(geom)));
-  if (hipMemset(c->dm[0], 0, 16 * V * geom->R * geom->nc) != hipSuccess ||
-      hipMemset(c->sp[0], 0, 8 * V * geom->R) != hipSuccess ||
-      hipMemset(c->jp, 0, 8 * V * 3) != hipSuccess)
+  // every transfer and clear of a context goes on its own (non-blocking)
+  // stream: the legacy null stream does not order against it
+  if (hipMemsetAsync(c->dm[0], 0, 16 * V * geom->R * geom->nc, c->stream) != hipSuccess ||
+      hipMemsetAsync(c->sp[0], 0, 8 * V * geom->R, c->stream) != hipSuccess ||
+      hipMemsetAsync(c->jp, 0, 8 * V * 3, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess)
     return cleanup(msa_fail(MSA_ERR_HIP, "initial clear failed"));
   pick_launch(c);
   pick_fused(c);
@@ -536,7 +539,8 @@ extern "C" int msa_download_state(msa_ctx *c, uint64_t *spin, uint64_t *d0, uint
   HIPCHK(hipStreamSynchronize(c->stream));
   if (spin) {
     std::vector<uint64_t> sb(g.R * V);
-    HIPCHK(hipMemcpy(sb.data(), c->sp[c->scur], 8 * g.R * V, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpyAsync(sb.data(), c->sp[c->scur], 8 * g.R * V, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     for (uint64_t s = 0; s < V; ++s) {
       uint32_t col;
       const uint64_t h = split_index(g, s, &col);
@@ -545,7 +549,8 @@ extern "C" int msa_download_state(msa_ctx *c, uint64_t *spin, uint64_t *d0, uint
   }
   if (d0 || d1) {
     std::vector<ulonglong2> db((uint64_t)g.R * g.nc * V);
-    HIPCHK(hipMemcpy(db.data(), c->dm[c->cur], 16 * db.size(), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpyAsync(db.data(), c->dm[c->cur], 16 * db.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     for (uint64_t s = 0; s < V; ++s) {
       uint32_t col;
       const uint64_t h = split_index(g, s, &col);
@@ -568,7 +573,8 @@ extern "C" int msa_download_couplings(msa_ctx *c, uint64_t *J) {
   const uint64_t V = g.V, H = g.H;
   HIPCHK(hipStreamSynchronize(c->stream));
   std::vector<uint64_t> jb(3 * V);
-  HIPCHK(hipMemcpy(jb.data(), c->jp, 8 * 3 * V, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpyAsync(jb.data(), c->jp, 8 * 3 * V, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
   for (uint64_t s = 0; s < V; ++s) {
     uint32_t col;
     const uint64_t h = split_index(g, s, &col);
@@ -604,7 +610,9 @@ extern "C" int msa_set_luts(msa_ctx *c, int32_t nlut, const msa_lut *luts) {
   c->nlut = 0;
   if (hipMalloc(&c->luts, sizeof(msa_lut) * 2 * (size_t)nlut) != hipSuccess)
     return msa_fail(MSA_ERR_NOMEM, "LUT table of %d pairs", nlut);
-  HIPCHK(hipMemcpy(c->luts, luts, sizeof(msa_lut) * 2 * (size_t)nlut, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpyAsync(c->luts, luts, sizeof(msa_lut) * 2 * (size_t)nlut, hipMemcpyHostToDevice,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
   c->nlut = nlut;
   return MSA_OK;
 }
@@ -635,8 +643,10 @@ extern "C" int msa_set_refresh(msa_ctx *c, int32_t nref, const msa_u32x4 *ctr,
   if (hipMalloc(&c->ctr, sizeof(uint4) * R * (size_t)nref) != hipSuccess ||
       hipMalloc(&c->key, sizeof(uint32_t) * R) != hipSuccess)
     return msa_fail(MSA_ERR_NOMEM, "refresh table of %d entries", nref);
-  HIPCHK(hipMemcpy(c->ctr, ctr, sizeof(uint4) * R * (size_t)nref, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->key, key, sizeof(uint32_t) * R, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpyAsync(c->ctr, ctr, sizeof(uint4) * R * (size_t)nref, hipMemcpyHostToDevice,
+                        c->stream));
+  HIPCHK(hipMemcpyAsync(c->key, key, sizeof(uint32_t) * R, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
   c->lut_idx.assign(lut_idx, lut_idx + nref);
   c->nref = nref;
   return MSA_OK;
@@ -769,6 +779,48 @@ extern "C" int msa_measure(msa_ctx *c, int32_t *unsat, int32_t *demon, int32_t *
     if (qlink && (rc = measure_one(c, 3, (int)(2 * p), (int)(2 * p + 1), qlink + 64 * p))) return rc;
   }
   return MSA_OK;
+}
+
+// ------------------------------------------------------- copy probe ------
+// Streaming copy of n 16-byte words (grid-stride, 16 B per lane per access).
+__global__ void __launch_bounds__(256) k_copy16(const ulonglong2 *__restrict__ src,
+                                                ulonglong2 *__restrict__ dst, uint64_t n) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) dst[k] = src[k];
+}
+
+extern "C" int msa_copy_probe(int device, int64_t bytes, int32_t iters, float *ms_per_copy) {
+  if (bytes < 4096 || iters < 1 || !ms_per_copy) return msa_fail(MSA_ERR_ARG, "bad copy probe arguments");
+  HIPCHK(hipSetDevice(device));
+  const uint64_t n = (uint64_t)bytes / 16;
+  ulonglong2 *a = nullptr, *b = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = MSA_OK;
+  if (hipMalloc(&a, n * 16) != hipSuccess || hipMalloc(&b, n * 16) != hipSuccess) {
+    rc = msa_fail(MSA_ERR_NOMEM, "copy probe buffers");
+  } else if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+             hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    rc = msa_fail(MSA_ERR_HIP, "copy probe stream");
+  } else {
+    (void)hipMemsetAsync(a, 1, n * 16, st);
+    const uint32_t grid = 256 * 32;
+    hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(256), 0, st, a, b, n);   // warm-up
+    (void)hipEventRecord(e0, st);
+    for (int32_t it = 0; it < iters; ++it)
+      hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(256), 0, st, a, b, n);
+    (void)hipEventRecord(e1, st);
+    if (hipEventSynchronize(e1) != hipSuccess) rc = msa_fail(MSA_ERR_HIP, "copy probe sync");
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    *ms_per_copy = ms / (float)iters;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (st) (void)hipStreamDestroy(st);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return rc;
 }
 
 // ---------------------------------------------------------- autotuner ----

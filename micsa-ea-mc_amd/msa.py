@@ -87,6 +87,7 @@ def hip():
         L.msa_batch_measure.argtypes = [bp, C.c_int32, i32p, i32p, i32p, i32p]
         L.msa_batch_download_state.argtypes = [bp, C.c_int32, u64p, u64p, u64p]
         L.msa_batch_remove.argtypes = [bp, C.c_int32]
+        L.msa_copy_probe.argtypes = [C.c_int, C.c_int64, C.c_int32, C.POINTER(C.c_float)]
         L.msa_autotune.argtypes = [vp, C.c_int32, i32p, i32p, i32p, C.POINTER(C.c_float)]
         L.msa_footprint_bytes.argtypes = [C.POINTER(Geom)]
         L.msa_footprint_bytes.restype = C.c_int64
@@ -205,6 +206,13 @@ class Lattice:
             out["q"] = q[:npair]
             out["qlink"] = ql[:npair]
         return out
+
+
+def copy_probe(nbytes, iters=20, device=0):
+    """ms per streaming copy of nbytes (reads nbytes, writes nbytes)."""
+    ms = C.c_float()
+    _check(hip().msa_copy_probe(device, nbytes, iters, C.byref(ms)))
+    return ms.value
 
 
 def footprint_bytes(L, R, nc):

@@ -91,6 +91,7 @@ def serve(requests, max_bytes, epoch=64, block=256, tables=None, keep_state=Fals
     done = {}
     updates = 0
     max_active = 0
+    max_bytes_used = 0
     while len(done) < len(reqs):
         now = clock() - t_start
         while nxt < len(reqs) and reqs[nxt].arrival <= now:
@@ -106,6 +107,7 @@ def serve(requests, max_bytes, epoch=64, block=256, tables=None, keep_state=Fals
             queue.pop(0)
             active[slot] = (req, clock() - t_start)
         max_active = max(max_active, len(active))
+        max_bytes_used = max(max_bytes_used, batch.bytes_in_use())
         if not active:
             if nxt < len(reqs):
                 time.sleep(max(0.0, min(0.01, reqs[nxt].arrival - (clock() - t_start))))
@@ -131,5 +133,6 @@ def serve(requests, max_bytes, epoch=64, block=256, tables=None, keep_state=Fals
                "spin_updates_per_ps": updates / makespan / 1e12,
                "latency_p50_s": float(np.percentile(lat, 50)),
                "latency_p99_s": float(np.percentile(lat, 99)),
-               "max_concurrent": max_active, "epoch": epoch}
+               "max_concurrent": max_active, "max_bytes_in_use": max_bytes_used,
+               "epoch": epoch}
     return done, summary

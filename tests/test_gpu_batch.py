@@ -98,6 +98,8 @@ def test_batch_admission_and_churn(pkg, orc):
     batch.remove(a)                               # retire one mid-run
     c = batch.add(job)                            # and admit a new one in its place
     assert c == a
+    t, _ = batch.progress(2)
+    assert t[c] == 0 and t[b] == 5                # the new lattice starts from zero
     batch.run(20)
     batch.sync()
     t, te = batch.progress(2)

@@ -197,3 +197,22 @@ def test_rnd_word_density(orc):
     frac = ones / (32.0 * n)
     sigma = np.sqrt(q * (1 - q) / (32.0 * n))
     assert abs(frac - q) < 5 * sigma
+
+
+def test_device_update_circuit_equals_reference_algebra(orc):
+    """The reduced circuit of msa::site_update (csrc/msa_device.hpp) against the
+    oracle's literal ISI algebra (main.cu:1037-1074), all 64 inputs."""
+    for npos in range(4):
+        for nneg in range(4):
+            for d in range(4):
+                j1, j2, j3, j4 = npos & 1, npos >> 1, nneg & 1, nneg >> 1
+                D0, D1 = d & 1, d >> 1
+                c1 = int((D0 + j1 + j3) in (1, 2))
+                c2 = D0 & j1 & j3
+                v0 = D1 ^ j2 ^ j4
+                v1 = int(D1 + j2 + j4 >= 2)
+                flip = v1 ^ c2 ^ (c1 & v0)
+                nd0 = (1 - (D0 ^ j1 ^ j3)) if flip else D0
+                nd1 = (v0 ^ c1) if flip else D1
+                r0, r1, rf = orc.isi(j1, j2, j3, j4, D0, D1)
+                assert (nd0, nd1, flip) == (r0 & 1, r1 & 1, rf & 1), (npos, nneg, d)
