@@ -10,12 +10,15 @@ timed region covers exactly --steps sweeps, bracketed by stream syncs and
 timed with HIP events on the library's stream.
 
 Prints one JSON line.  `value` = spin updates per picosecond
-(64 lanes x R x V x steps / seconds / 1e12).  `roofline` compares the
-algorithmic HBM bytes of the timed sweeps, (48R+24) V per sweep and
-(32R+24) V per sweep with a fused refresh (its demon read is elided), with
-the MI355X HBM peak of 8 TB/s.  `cpu_baseline` times the oracle's
-word-level CPU sweep (tests/oracle_lib.py) on a bounded sample of the same
-lattice on this host.
+(64 lanes x R x V x steps / seconds / 1e12).  `roofline` is for the
+dominant kernel, the sweep kernel of sweeps without a refresh: its
+algorithmic HBM bytes per launch, (48R+24) V, over its average launch time,
+measured with HIP events on the library's stream over a separate run of
+sweeps that all use it, against the MI355X HBM peak of 8 TB/s.  `traffic`
+(HBM bytes per launch from counters) is read from the profile summary in
+profiles/ when one exists for the same kernel and workload.
+`cpu_baseline` times the oracle's word-level CPU sweep (tests/oracle_lib.py)
+on a bounded sample of the same lattice on this host.
 """
 import argparse
 import json
@@ -40,6 +43,34 @@ def load_package():
     sys.modules["micsa_ea_mc_amd"] = mod
     spec.loader.exec_module(mod)
     return mod
+
+
+KERNELS = {0: "k_halfsweep", 1: "k_sweep_fused<%d>", 2: "k_sweep_fused2<%d>"}
+
+
+def kernel_name(mode, R):
+    name = KERNELS[mode]
+    return name % R if "%d" in name else name
+
+
+def read_traffic(kernel, L, R):
+    """HBM bytes per launch measured by counters for this kernel and lattice,
+    from profiles/traffic_*.txt (key=value lines), or None."""
+    d = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(d):
+        return None
+    for fn in sorted(os.listdir(d)):
+        if not (fn.startswith("traffic_") and fn.endswith(".txt")):
+            continue
+        kv = {}
+        with open(os.path.join(d, fn)) as f:
+            for ln in f:
+                if "=" in ln and not ln.startswith("#"):
+                    k, v = ln.split("=", 1)
+                    kv[k.strip()] = v.strip()
+        if kv.get("kernel") == kernel and kv.get("L") == str(L) and kv.get("R") == str(R):
+            return int(kv["hbm_bytes_per_launch"])
+    return None
 
 
 def alg_bytes(V, R, nsweeps, nrefresh):
@@ -221,6 +252,9 @@ def main():
     lat.sync()
     wall = time.perf_counter() - w0
     m = lat.measure()
+    # the dominant kernel alone: sweeps without a refresh
+    ksweeps = min(args.steps, 1024)
+    ms_k = lat.run_timed(args.warmup + args.steps, ksweeps, 0)
     lat.close()
 
     t0 = args.warmup
@@ -228,9 +262,12 @@ def main():
     sec = ms * 1e-3
     updates = 64.0 * R * V * args.steps
     value = updates / sec / 1e12
-    nbytes = alg_bytes(V, R, args.steps, nref)
-    achieved = nbytes / sec / 1e9
-    launches = 2 * nc * args.steps
+    per_sweep = (2 if tiling[0] == 0 else 1) * nc
+    launches_k = per_sweep * ksweeps
+    kbytes = alg_bytes(V, R, ksweeps, 0) / launches_k
+    avg_s = ms_k * 1e-3 / launches_k
+    achieved = kbytes / avg_s / 1e9
+    kname = kernel_name(tiling[0], R)
     out = {
         "metric": "spin_updates_per_ps",
         "value": round(value, 4),
@@ -250,7 +287,7 @@ def main():
                         "refresh every %d sweeps" % (L, R, nc, every),
             "L": L, "R": R, "nc": nc, "lanes": 64, "refresh_every": every,
             "sweeps_timed": args.steps, "refreshes_timed": nref,
-            "kernel_mode": "one-pass" if tiling[0] else "two-pass",
+            "kernel_mode": ["two-pass", "one-pass", "one-pass, single-load"][tiling[0]],
             "tile_rows_planes": [tiling[1], tiling[2]],
         },
         "roofline": {
@@ -259,10 +296,12 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
-            "kernel": "k_halfsweep",
-            "avg_launch_us": round(ms * 1e3 / launches, 3),
-            "alg_bytes_per_launch": int(nbytes / launches),
+            "traffic": read_traffic(kname, L, R),
+            "kernel": kname,
+            "avg_launch_us": round(avg_s * 1e6, 3),
+            "launches_timed": launches_k,
+            "alg_bytes_per_launch": int(kbytes),
+            "timed_region_alg_GBs": round(alg_bytes(V, R, args.steps, nref) / sec / 1e9, 1),
         },
         "wall_s": round(wall, 4),
         "mean_energy_per_site": round(float(-m["energy"].mean()) / V, 6),

@@ -126,9 +126,11 @@ int msa_measure(msa_ctx *ctx, int32_t *unsat, int32_t *demon, int32_t *q, int32_
 /* Kernel choice for sweeps without a refresh (no reference counterpart; the
  * autotuner and the batched form set it).  mode 0: two kernels per sweep,
  * one per colour.  mode 1: one kernel per sweep whose blocks own ty rows x
- * tz planes (ty, tz dividing L; 0,0 = built-in default).  Sweeps with a
- * refresh always use the two-kernel form.  Results are identical in every
- * mode; only the speed differs. */
+ * tz planes (ty, tz dividing L; 0,0 = built-in default).  mode 2: the same
+ * tiles, each word loaded once per block (register plane window, LDS
+ * exchange) and tiles placed per XCD.  Sweeps with a refresh always use the
+ * two-kernel form.  Results are identical in every mode; only the speed
+ * differs. */
 int msa_set_tiling(msa_ctx *ctx, int32_t mode, int32_t ty, int32_t tz);
 int msa_get_tiling(msa_ctx *ctx, int32_t *mode, int32_t *ty, int32_t *tz);
 

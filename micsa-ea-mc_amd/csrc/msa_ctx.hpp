@@ -25,7 +25,7 @@ struct msa_ctx {
   std::vector<int32_t> lut_idx;           // host copy, indexes luts
   int32_t *meas = nullptr;                // [64]
   dim3 blk{1, 1, 1}, grd{1, 1, 1};        // two-pass kernel launch
-  int fused = 0;                          // use the one-pass kernel for sweeps without refresh
+  int fused = 0;                          // sweeps without refresh: 0 two-pass, 1/2 one-pass kernel
   uint32_t fty = 0, ftz = 0, fny = 0;     // its tile: rows, planes, thread rows
   int fyhalo = 1;
 };

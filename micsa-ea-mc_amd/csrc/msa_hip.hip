@@ -236,6 +236,157 @@ __global__ void __launch_bounds__(1024) k_sweep_fused(FusedArgs a) {
   }
 }
 
+// Tile index of a block of the one-pass kernels.  Blocks are dispatched to
+// the 8 XCDs round-robin (block b on XCD b mod 8); with `xcd` set, XCD x gets
+// the x-th contiguous run of tiles in (plane slab, row tile) order, so tiles
+// that share halo rows sit on the same XCD and meet in its L2.
+__device__ __forceinline__ void tile_of(uint32_t b, uint32_t nty, uint32_t ntz, int xcd,
+                                        uint32_t &ty_idx, uint32_t &tz_idx) {
+  const uint32_t n = nty * ntz;
+  uint32_t t = b;
+  if (xcd && (n % 8u) == 0) t = (b % 8u) * (n / 8u) + b / 8u;
+  ty_idx = t % nty;
+  tz_idx = t / nty;
+}
+
+// One sweep in one pass, second form.  Same wavefront as k_sweep_fused, but
+// every word a block needs is loaded from global memory once per block:
+//   * odd-colour spins of its own column ride a three-plane register window
+//     (z-1, z, z+1) and reach in-plane neighbours through one LDS plane;
+//   * the thread's own J words of a plane stay in registers for the later
+//     steps that use them (col-1 J for the odd update one step later,
+//     col-0 J_z for the -z bond two steps later);
+//   * new colour-0 values are exchanged through a double LDS buffer.
+// Two barriers per step.  Tiles are placed XCD-aware (tile_of).
+// At R = 4 the LDS bound (three planes in 64 KiB) already limits a block to
+// 682 threads, so the launch bound is lowered to leave more registers.
+template <int R>
+__global__ void __launch_bounds__(R >= 4 ? 768 : 1024) k_sweep_fused2(FusedArgs a) {
+  extern __shared__ uint64_t s_lds[];
+  const Geo g = a.g;
+  const uint32_t Lh = g.Lh, L = g.L;
+  const uint64_t H = g.H;
+  const uint32_t i = threadIdx.x % Lh, row = threadIdx.x / Lh;
+  uint32_t tyi, tzi;
+  tile_of(blockIdx.x, L / a.ty, L / a.tz, 1, tyi, tzi);
+  const uint32_t y0 = tyi * a.ty, z0 = tzi * a.tz;
+  const bool yh = a.yhalo != 0;
+  const uint32_t y = yh ? msa::wrapi((int)(y0 + row) - 1, L) : row;
+  const bool own = yh ? (row >= 1 && row <= a.ty) : true;
+  const uint32_t rm = yh ? (row == 0 ? 0 : row - 1) : (row == 0 ? L - 1 : row - 1);
+  const uint32_t rp = yh ? (row + 1 == a.ny ? row : row + 1) : (row + 1 == L ? 0 : row + 1);
+  const uint32_t ym = msa::wrapi((int)y - 1, L), yp = msa::wrapi((int)y + 1, L);
+  const uint64_t slot = ((uint64_t)row * Lh + i) * R;          // this thread's LDS slot
+  const uint64_t plane = (uint64_t)a.ny * Lh * R;              // one LDS plane
+  uint64_t *S1p = s_lds;                                       // odd spins, plane s
+  uint64_t *Eb = s_lds + plane;                                // new even, planes s-1 / s
+  int dx, dy, dz;
+  msa::layer_dir(a.layer, dx, dy, dz);
+  const uint64_t *J = a.jp;
+  const uint64_t lbase0 = (uint64_t)(a.layer * 2 + 0) * H, lbase1 = (uint64_t)(a.layer * 2 + 1) * H;
+  const uint64_t rstride = (uint64_t)g.nc * 2 * H;
+  const uint64_t LhL = (uint64_t)Lh * L;
+  const uint64_t ybase = (uint64_t)Lh * y, ymb = (uint64_t)Lh * ym, ypb = (uint64_t)Lh * yp;
+
+  uint64_t Sm[R], S0w[R], Sp[R], Em2[R], Em1[R], E0[R];
+  // prime: odd spins of planes z0-2, z0-1; col-1 J_z of plane z0-2
+  {
+    const uint32_t za = msa::wrapi((int)z0 - 2, L), zb = msa::wrapi((int)z0 - 1, L);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const uint64_t *S1 = a.sp_in + (uint64_t)(2 * r + 1) * H;
+      Sm[r] = S1[i + ybase + LhL * za];
+      S0w[r] = S1[i + ybase + LhL * zb];
+      Em2[r] = Em1[r] = E0[r] = 0;
+    }
+  }
+  uint64_t j1z_prev = J[5 * H + i + ybase + LhL * msa::wrapi((int)z0 - 2, L)];
+  uint64_t j1x_prev = 0, j1y_prev = 0;       // col-1 J of plane s-1
+  uint64_t j0z_m1 = 0, j0z_m2 = 0;           // col-0 J_z of planes s-1, s-2
+
+  const int nstep = (int)a.tz + 2;
+  for (int k = 0; k < nstep; ++k) {
+    const int s = (int)z0 - 1 + k;
+    const uint32_t z = msa::wrapi(s, L);
+    const uint32_t zn = msa::wrapi(s + 1, L);
+    const uint64_t zb = LhL * z;
+    const uint64_t h = i + ybase + zb;
+    // ---- loads of this step: odd spins of plane s+1, own words of plane s
+#pragma unroll
+    for (int r = 0; r < R; ++r) Sp[r] = a.sp_in[(uint64_t)(2 * r + 1) * H + i + ybase + LhL * zn];
+    const uint64_t j0x = J[h], j0y = J[H + h], j0z = J[2 * H + h];
+    const uint64_t j1x = J[3 * H + h], j1y = J[4 * H + h], j1z = J[5 * H + h];
+#pragma unroll
+    for (int r = 0; r < R; ++r) S1p[slot + r] = S0w[r];
+    __syncthreads();
+    // ---- colour 0, plane s
+    {
+      const uint32_t off = (y + z) & 1u;
+      const uint32_t ip = off ? msa::wrapi((int)i + 1, Lh) : i, im = off ? i : msa::wrapi((int)i - 1, Lh);
+      const uint64_t J3 = J[3 * H + im + ybase + zb], J4 = J[4 * H + i + ymb + zb];
+      const bool write = own && k >= 1 && k <= (int)a.tz;
+      const uint64_t hw = msa::walk_dest(g, 0, dx, dy, dz, i, y, z);
+      uint64_t *Ew = Eb + (uint64_t)(k & 1) * plane;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const ulonglong2 d = a.dm_in[r * rstride + lbase0 + h];
+        uint64_t D0 = d.x, D1 = d.y;
+        const uint64_t nxp = S1p[((uint64_t)row * Lh + ip) * R + r];
+        const uint64_t nxm = S1p[((uint64_t)row * Lh + im) * R + r];
+        const uint64_t nyp = yh && row + 1 == a.ny ? a.sp_in[(uint64_t)(2 * r + 1) * H + i + ypb + zb]
+                                                   : S1p[((uint64_t)rp * Lh + i) * R + r];
+        const uint64_t nym = yh && row == 0 ? a.sp_in[(uint64_t)(2 * r + 1) * H + i + ymb + zb]
+                                            : S1p[((uint64_t)rm * Lh + i) * R + r];
+        E0[r] = msa::site_update(a.sp_in[(uint64_t)(2 * r) * H + h], nxp, nyp, Sp[r], nxm, nym, Sm[r],
+                                 j0x, j0y, j0z, J3, J4, j1z_prev, D0, D1);
+        Ew[slot + r] = E0[r];
+        if (write) {
+          a.sp_out[(uint64_t)(2 * r) * H + h] = E0[r];
+          a.dm_out[r * rstride + lbase0 + hw] = make_ulonglong2(D0, D1);
+        }
+      }
+    }
+    __syncthreads();
+    // ---- colour 1, plane s-1
+    if (own && k >= 2) {
+      const uint32_t zo = msa::wrapi(s - 1, L);
+      const uint64_t zob = LhL * zo;
+      const uint32_t off = (y + zo + 1) & 1u;
+      const uint32_t ip = off ? msa::wrapi((int)i + 1, Lh) : i, im = off ? i : msa::wrapi((int)i - 1, Lh);
+      const uint64_t ho = i + ybase + zob;
+      const uint64_t J3 = J[im + ybase + zob], J4 = J[H + i + ymb + zob];
+      const uint64_t hw = msa::walk_dest(g, 1, dx, dy, dz, i, y, zo);
+      const uint64_t *Er = Eb + (uint64_t)((k - 1) & 1) * plane;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const ulonglong2 d = a.dm_in[r * rstride + lbase1 + ho];
+        uint64_t D0 = d.x, D1 = d.y;
+        const uint64_t nxp = Er[((uint64_t)row * Lh + ip) * R + r];
+        const uint64_t nxm = Er[((uint64_t)row * Lh + im) * R + r];
+        const uint64_t nyp = Er[((uint64_t)rp * Lh + i) * R + r];
+        const uint64_t nym = Er[((uint64_t)rm * Lh + i) * R + r];
+        const uint64_t sn = msa::site_update(Sm[r], nxp, nyp, E0[r], nxm, nym, Em2[r], j1x_prev,
+                                             j1y_prev, j1z_prev, J3, J4, j0z_m2, D0, D1);
+        a.sp_out[(uint64_t)(2 * r + 1) * H + ho] = sn;
+        a.dm_out[r * rstride + lbase1 + hw] = make_ulonglong2(D0, D1);
+      }
+    }
+    // ---- rotate the windows
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      Sm[r] = S0w[r];
+      S0w[r] = Sp[r];
+      Em2[r] = Em1[r];
+      Em1[r] = E0[r];
+    }
+    j1x_prev = j1x;
+    j1y_prev = j1y;
+    j1z_prev = j1z;
+    j0z_m2 = j0z_m1;
+    j0z_m1 = j0z;
+  }
+}
+
 struct MeasArgs {
   const uint64_t *sp;
   const uint64_t *jp;
@@ -371,11 +522,12 @@ static int fused_tile_ok(const Geo &g, uint32_t ty, uint32_t tz) {
   const uint32_t ny = ty == g.L ? g.L : ty + 2;
   if ((uint64_t)g.Lh * ny > 1024) return 0;
   if ((uint64_t)3 * ny * g.Lh * g.R * 8 > 65536) return 0;
+  if (g.R >= 4 && (uint64_t)g.Lh * ny > 768) return 0;   // launch bound of the R = 4 kernels
   return 1;
 }
 
-static void set_fused(msa_ctx *c, uint32_t ty, uint32_t tz) {
-  c->fused = 1;
+static void set_fused(msa_ctx *c, int mode, uint32_t ty, uint32_t tz) {
+  c->fused = mode;
   c->fty = ty;
   c->ftz = tz;
   c->fyhalo = ty != c->g.L;
@@ -399,7 +551,7 @@ static void pick_fused(msa_ctx *c) {
       const double score = fill / waste;
       if (score > best) { best = score; best_ty = ty; best_tz = tz; }
     }
-  if (best_ty) set_fused(c, best_ty, best_tz);
+  if (best_ty) set_fused(c, 2, best_ty, best_tz);
 }
 
 extern "C" int msa_create(int device, const msa_geom *geom, msa_ctx **out) {
@@ -472,15 +624,17 @@ extern "C" int msa_set_tiling(msa_ctx *c, int32_t mode, int32_t ty, int32_t tz) 
     c->fused = 0;
     return MSA_OK;
   }
-  if (mode != 1) return msa_fail(MSA_ERR_ARG, "mode=%d (0 two-pass, 1 one-pass)", mode);
+  if (mode != 1 && mode != 2)
+    return msa_fail(MSA_ERR_ARG, "mode=%d (0 two-pass, 1 and 2 one-pass)", mode);
   if (ty == 0 && tz == 0) {
     pick_fused(c);
     if (!c->fused) return msa_fail(MSA_ERR_ARG, "no one-pass tile for L=%u R=%u", c->g.L, c->g.R);
+    c->fused = mode;
     return MSA_OK;
   }
   if (!fused_tile_ok(c->g, (uint32_t)ty, (uint32_t)tz))
     return msa_fail(MSA_ERR_ARG, "one-pass tile ty=%d tz=%d invalid for L=%u R=%u", ty, tz, c->g.L, c->g.R);
-  set_fused(c, (uint32_t)ty, (uint32_t)tz);
+  set_fused(c, mode, (uint32_t)ty, (uint32_t)tz);
   return MSA_OK;
 }
 
@@ -653,9 +807,18 @@ extern "C" int msa_set_refresh(msa_ctx *c, int32_t nref, const msa_u32x4 *ctr,
 }
 
 static void launch_fused(msa_ctx *c, const FusedArgs &fa) {
-  const dim3 grid(c->g.L / c->fty, c->g.L / c->ftz);
   const dim3 block(c->g.Lh * c->fny);
   const size_t lds = (size_t)3 * c->fny * c->g.Lh * c->g.R * 8;
+  if (c->fused == 2) {
+    const dim3 grid((c->g.L / c->fty) * (c->g.L / c->ftz));
+    switch (c->g.R) {
+      case 1: hipLaunchKernelGGL(k_sweep_fused2<1>, grid, block, lds, c->stream, fa); break;
+      case 2: hipLaunchKernelGGL(k_sweep_fused2<2>, grid, block, lds, c->stream, fa); break;
+      default: hipLaunchKernelGGL(k_sweep_fused2<4>, grid, block, lds, c->stream, fa); break;
+    }
+    return;
+  }
+  const dim3 grid(c->g.L / c->fty, c->g.L / c->ftz);
   switch (c->g.R) {
     case 1: hipLaunchKernelGGL(k_sweep_fused<1>, grid, block, lds, c->stream, fa); break;
     case 2: hipLaunchKernelGGL(k_sweep_fused<2>, grid, block, lds, c->stream, fa); break;
@@ -846,13 +1009,14 @@ extern "C" int msa_autotune(msa_ctx *c, int32_t sweeps, int32_t *mode, int32_t *
       if (ms_per_sweep) *ms_per_sweep = e.ms;
       return MSA_OK;
     }
-  // candidates: the two-pass form and every valid one-pass tile with slabs of
-  // at most 32 planes
+  // candidates: the two-pass form and both one-pass kernels on every valid
+  // tile with slabs of at most 32 planes
   struct Cand { int32_t mode, ty, tz; };
   std::vector<Cand> cands{{0, 0, 0}};
   for (uint32_t y = 1; y <= c->g.L; ++y)
     for (uint32_t z = 2; z <= c->g.L && z <= 32; ++z)
-      if (fused_tile_ok(c->g, y, z)) cands.push_back({1, (int32_t)y, (int32_t)z});
+      if (fused_tile_ok(c->g, y, z))
+        for (int32_t m = 1; m <= 2; ++m) cands.push_back({m, (int32_t)y, (int32_t)z});
   msa_ctx *t = nullptr;
   int rc = msa_create(c->device, &c->geom, &t);
   if (rc) return rc;

@@ -133,15 +133,18 @@ def test_errors_are_reported(pkg):
     (20, 2, 6, 5, 10),      # all six walk directions
     (40, 1, 2, 40, 8),
 ])
-def test_one_pass_sweep_matches_oracle(pkg, orc, L, R, nc, ty, tz):
-    print("case L=%d R=%d nc=%d tile %dx%d" % (L, R, nc, ty, tz), flush=True)
+@pytest.mark.parametrize("mode", [1, 2])
+def test_one_pass_sweep_matches_oracle(pkg, orc, L, R, nc, ty, tz, mode):
+    """Both one-pass kernels; the 16- and 8-tile cases also take the XCD
+    placement of mode 2 (tile count a multiple of 8)."""
+    print("case L=%d R=%d nc=%d tile %dx%d mode %d" % (L, R, nc, ty, tz, mode), flush=True)
     o = random_state(L, R, nc, 4242 + L)
     every, niter = 3, 7
     times = list(range(0, niter, every))
     ctr, keys, raw, luts = oracle_schedule(orc, R, times, [0.8] * len(times), jobid=1)
     dev = pkg.Lattice(L, R, nc)
-    dev.set_tiling(1, ty, tz)
-    assert dev.tiling() == (1, ty, tz)
+    dev.set_tiling(mode, ty, tz)
+    assert dev.tiling() == (mode, ty, tz)
     dev.upload(o.J, o.S, o.D0, o.D1)
     dev.set_luts(raw)
     dev.set_refresh(ctr, keys, np.arange(len(times), dtype=np.int32))
@@ -160,7 +163,7 @@ def test_default_tiling_matches_two_pass(pkg):
     L, R = 32, 2
     a = pkg.Lattice(L, R, 1)
     b = pkg.Lattice(L, R, 1)
-    assert a.tiling()[0] == 1
+    assert a.tiling()[0] == 2
     b.set_tiling(0)
     for d in (a, b):
         d.init_philox(99)
@@ -172,3 +175,24 @@ def test_default_tiling_matches_two_pass(pkg):
         assert np.array_equal(x, y)
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("L,R,ty,tz", [(96, 2, 16, 8), (64, 1, 8, 16), (48, 4, 6, 12)])
+def test_one_pass_kernels_agree_at_larger_sizes(pkg, L, R, ty, tz):
+    """Modes 1 and 2 against the two-pass form on bigger lattices (many
+    tiles, XCD placement active), 6 sweeps from a Philox state."""
+    print("case L=%d R=%d tile %dx%d" % (L, R, ty, tz), flush=True)
+    devs = [pkg.Lattice(L, R, 1) for _ in range(3)]
+    rng = np.random.default_rng(L)
+    d0, d1 = (rng.integers(0, 2 ** 64, size=(R, 1, L ** 3), dtype=np.uint64) for _ in range(2))
+    for m, d in enumerate(devs):
+        d.set_tiling(m, ty if m else 0, tz if m else 0)
+        d.init_philox(1234)
+        d.upload(d.couplings(), d.download()[0], d0, d1)
+        d.run(0, 6, 0)
+    ref = devs[0].download()
+    for d in devs[1:]:
+        for x, y in zip(d.download(), ref):
+            assert np.array_equal(x, y)
+    for d in devs:
+        d.close()
