@@ -44,6 +44,37 @@ __device__ __forceinline__ uint64_t walk_dest(const Geo &g, int col, int dx, int
   return wi + (uint64_t)g.Lh * (wy + (uint64_t)g.L * wz);
 }
 
+// Access at a 32-bit byte offset from a wave-uniform base: the base stays in
+// scalar registers and each access costs one offset VGPR (the saddr form of
+// global_load / global_store).  Callers guarantee offset < 2^32.
+template <class T>
+__device__ __forceinline__ T ldo(const T *base, uint32_t boff) {
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + boff);
+}
+template <class T>
+__device__ __forceinline__ void sto(T *base, uint32_t boff, T v) {
+  *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + boff) = v;
+}
+
+// Streaming forms for data a sweep touches once: non-temporal hint, so the
+// lines do not displace reused data (spins and J shared with neighbouring
+// tiles) from L2 and the last-level cache.
+typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ ulonglong2 ldo_stream(const ulonglong2 *base, uint32_t boff) {
+  const u64x2 v = __builtin_nontemporal_load(
+      reinterpret_cast<const u64x2 *>(reinterpret_cast<const char *>(base) + boff));
+  return make_ulonglong2(v.x, v.y);
+}
+__device__ __forceinline__ void sto_stream(uint64_t *base, uint32_t boff, uint64_t v) {
+  __builtin_nontemporal_store(v, reinterpret_cast<uint64_t *>(reinterpret_cast<char *>(base) + boff));
+}
+__device__ __forceinline__ void sto_stream(ulonglong2 *base, uint32_t boff, uint64_t x, uint64_t y) {
+  u64x2 v;
+  v.x = x;
+  v.y = y;
+  __builtin_nontemporal_store(v, reinterpret_cast<u64x2 *>(reinterpret_cast<char *>(base) + boff));
+}
+
 // ---------------------------------------------------------------- Philox --
 // Philox4x32-10 with the round and key schedule of main.cu:562-595
 // (Random123's philox4x32 with 10 rounds).

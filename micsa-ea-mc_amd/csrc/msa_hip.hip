@@ -257,11 +257,20 @@ __device__ __forceinline__ void tile_of(uint32_t b, uint32_t nty, uint32_t ntz, 
 //     steps that use them (col-1 J for the odd update one step later,
 //     col-0 J_z for the -z bond two steps later);
 //   * new colour-0 values are exchanged through a double LDS buffer.
-// Two barriers per step.  Tiles are placed XCD-aware (tile_of).
-// At R = 4 the LDS bound (three planes in 64 KiB) already limits a block to
-// 682 threads, so the launch bound is lowered to leave more registers.
+// Two barriers per step; the loads of each colour are issued right after the
+// barrier that precedes it (all waves of the block then fetch the same lines
+// together).  Demons are read and spins and demons written with the streaming
+// hint: each is touched once per sweep, while spins and J are re-read by the
+// neighbouring tiles' halos.  Tiles are placed XCD-aware (tile_of).  Offsets
+// are 32-bit byte offsets from uniform bases (every buffer below 4 GiB,
+// checked by the host).  At R = 4 the LDS bound (three planes in 64 KiB)
+// already limits a block to 682 threads, so the launch bound is lowered to
+// leave more registers.
 template <int R>
 __global__ void __launch_bounds__(R >= 4 ? 768 : 1024) k_sweep_fused2(FusedArgs a) {
+  using msa::ldo;
+  using msa::ldo_stream;
+  using msa::sto_stream;
   extern __shared__ uint64_t s_lds[];
   const Geo g = a.g;
   const uint32_t Lh = g.Lh, L = g.L;
@@ -276,99 +285,106 @@ __global__ void __launch_bounds__(R >= 4 ? 768 : 1024) k_sweep_fused2(FusedArgs 
   const uint32_t rm = yh ? (row == 0 ? 0 : row - 1) : (row == 0 ? L - 1 : row - 1);
   const uint32_t rp = yh ? (row + 1 == a.ny ? row : row + 1) : (row + 1 == L ? 0 : row + 1);
   const uint32_t ym = msa::wrapi((int)y - 1, L), yp = msa::wrapi((int)y + 1, L);
-  const uint64_t slot = ((uint64_t)row * Lh + i) * R;          // this thread's LDS slot
-  const uint64_t plane = (uint64_t)a.ny * Lh * R;              // one LDS plane
-  uint64_t *S1p = s_lds;                                       // odd spins, plane s
-  uint64_t *Eb = s_lds + plane;                                // new even, planes s-1 / s
+  const uint32_t slot = (row * Lh + i) * R;              // this thread's LDS slot
+  const uint32_t plane = a.ny * Lh * R;                  // one LDS plane
+  uint64_t *S1p = s_lds;                                 // odd spins, plane s
+  uint64_t *Eb = s_lds + plane;                          // new even, planes s-1 / s
   int dx, dy, dz;
   msa::layer_dir(a.layer, dx, dy, dz);
-  const uint64_t *J = a.jp;
-  const uint64_t lbase0 = (uint64_t)(a.layer * 2 + 0) * H, lbase1 = (uint64_t)(a.layer * 2 + 1) * H;
+  const uint64_t *Jx0 = a.jp, *Jy0 = a.jp + H, *Jz0 = a.jp + 2 * H;
+  const uint64_t *Jx1 = a.jp + 3 * H, *Jy1 = a.jp + 4 * H, *Jz1 = a.jp + 5 * H;
   const uint64_t rstride = (uint64_t)g.nc * 2 * H;
-  const uint64_t LhL = (uint64_t)Lh * L;
-  const uint64_t ybase = (uint64_t)Lh * y, ymb = (uint64_t)Lh * ym, ypb = (uint64_t)Lh * yp;
+  const ulonglong2 *Din0 = a.dm_in + (uint64_t)(a.layer * 2) * H, *Din1 = Din0 + H;
+  ulonglong2 *Dout0 = a.dm_out + (uint64_t)(a.layer * 2) * H, *Dout1 = Dout0 + H;
+  const uint32_t LhL = Lh * L;
+  const uint32_t yb = Lh * y, ymb = Lh * ym, ypb = Lh * yp;
+  const bool hrow = yh && (row == 0 || row + 1 == a.ny);   // a halo row: one y neighbour outside
+  const uint32_t hrb = row == 0 ? ymb : ypb;
 
   uint64_t Sm[R], S0w[R], Sp[R], Em2[R], Em1[R], E0[R];
   // prime: odd spins of planes z0-2, z0-1; col-1 J_z of plane z0-2
-  {
-    const uint32_t za = msa::wrapi((int)z0 - 2, L), zb = msa::wrapi((int)z0 - 1, L);
+  const uint32_t hza = i + yb + LhL * msa::wrapi((int)z0 - 2, L);
+  const uint32_t hzb = i + yb + LhL * msa::wrapi((int)z0 - 1, L);
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const uint64_t *S1 = a.sp_in + (uint64_t)(2 * r + 1) * H;
-      Sm[r] = S1[i + ybase + LhL * za];
-      S0w[r] = S1[i + ybase + LhL * zb];
-      Em2[r] = Em1[r] = E0[r] = 0;
-    }
+  for (int r = 0; r < R; ++r) {
+    const uint64_t *S1 = a.sp_in + (uint64_t)(2 * r + 1) * H;
+    Sm[r] = ldo(S1, hza * 8u);
+    S0w[r] = ldo(S1, hzb * 8u);
+    Em2[r] = Em1[r] = E0[r] = 0;
   }
-  uint64_t j1z_prev = J[5 * H + i + ybase + LhL * msa::wrapi((int)z0 - 2, L)];
+  uint64_t j1z_prev = ldo(Jz1, hza * 8u);
   uint64_t j1x_prev = 0, j1y_prev = 0;       // col-1 J of plane s-1
   uint64_t j0z_m1 = 0, j0z_m2 = 0;           // col-0 J_z of planes s-1, s-2
 
   const int nstep = (int)a.tz + 2;
   for (int k = 0; k < nstep; ++k) {
     const int s = (int)z0 - 1 + k;
-    const uint32_t z = msa::wrapi(s, L);
-    const uint32_t zn = msa::wrapi(s + 1, L);
-    const uint64_t zb = LhL * z;
-    const uint64_t h = i + ybase + zb;
-    // ---- loads of this step: odd spins of plane s+1, own words of plane s
+    const uint32_t z = msa::wrapi(s, L), zo = msa::wrapi(s - 1, L), zn = msa::wrapi(s + 1, L);
+    const uint32_t h = i + yb + LhL * z, ho = i + yb + LhL * zo, hn = i + yb + LhL * zn;
+    // x parity of the colour-0 sites of plane s, (y + s) & 1, equals that of
+    // the colour-1 sites of plane s-1, (y + s - 1 + 1) & 1 (L is even), so
+    // both updates of a step share the x neighbours ip / im
+    const uint32_t off = (y + z) & 1u;
+    const uint32_t ip = off ? msa::wrapi((int)i + 1, Lh) : i, im = off ? i : msa::wrapi((int)i - 1, Lh);
+    // ---- the step's own column: odd spins of plane s+1, J of plane s
 #pragma unroll
-    for (int r = 0; r < R; ++r) Sp[r] = a.sp_in[(uint64_t)(2 * r + 1) * H + i + ybase + LhL * zn];
-    const uint64_t j0x = J[h], j0y = J[H + h], j0z = J[2 * H + h];
-    const uint64_t j1x = J[3 * H + h], j1y = J[4 * H + h], j1z = J[5 * H + h];
+    for (int r = 0; r < R; ++r) Sp[r] = ldo(a.sp_in + (uint64_t)(2 * r + 1) * H, hn * 8u);
+    const uint64_t j0x = ldo(Jx0, h * 8u), j0y = ldo(Jy0, h * 8u), j0z = ldo(Jz0, h * 8u);
+    const uint64_t j1x = ldo(Jx1, h * 8u), j1y = ldo(Jy1, h * 8u), j1z = ldo(Jz1, h * 8u);
+    const uint64_t c0J3 = ldo(Jx1, (im + yb + LhL * z) * 8u), c0J4 = ldo(Jy1, (i + ymb + LhL * z) * 8u);
 #pragma unroll
     for (int r = 0; r < R; ++r) S1p[slot + r] = S0w[r];
     __syncthreads();
     // ---- colour 0, plane s
     {
-      const uint32_t off = (y + z) & 1u;
-      const uint32_t ip = off ? msa::wrapi((int)i + 1, Lh) : i, im = off ? i : msa::wrapi((int)i - 1, Lh);
-      const uint64_t J3 = J[3 * H + im + ybase + zb], J4 = J[4 * H + i + ymb + zb];
-      const bool write = own && k >= 1 && k <= (int)a.tz;
-      const uint64_t hw = msa::walk_dest(g, 0, dx, dy, dz, i, y, z);
-      uint64_t *Ew = Eb + (uint64_t)(k & 1) * plane;
+      uint64_t S0v[R], Hv[R];
+      ulonglong2 dv[R];
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        const ulonglong2 d = a.dm_in[r * rstride + lbase0 + h];
-        uint64_t D0 = d.x, D1 = d.y;
-        const uint64_t nxp = S1p[((uint64_t)row * Lh + ip) * R + r];
-        const uint64_t nxm = S1p[((uint64_t)row * Lh + im) * R + r];
-        const uint64_t nyp = yh && row + 1 == a.ny ? a.sp_in[(uint64_t)(2 * r + 1) * H + i + ypb + zb]
-                                                   : S1p[((uint64_t)rp * Lh + i) * R + r];
-        const uint64_t nym = yh && row == 0 ? a.sp_in[(uint64_t)(2 * r + 1) * H + i + ymb + zb]
-                                            : S1p[((uint64_t)rm * Lh + i) * R + r];
-        E0[r] = msa::site_update(a.sp_in[(uint64_t)(2 * r) * H + h], nxp, nyp, Sp[r], nxm, nym, Sm[r],
-                                 j0x, j0y, j0z, J3, J4, j1z_prev, D0, D1);
+        S0v[r] = ldo(a.sp_in + (uint64_t)(2 * r) * H, h * 8u);
+        dv[r] = ldo_stream(Din0 + r * rstride, h * 16u);
+        Hv[r] = hrow ? ldo(a.sp_in + (uint64_t)(2 * r + 1) * H, (i + hrb + LhL * z) * 8u) : 0;
+      }
+      const bool write = own && k >= 1 && k <= (int)a.tz;
+      const uint32_t hw = (uint32_t)msa::walk_dest(g, 0, dx, dy, dz, i, y, z);
+      uint64_t *Ew = Eb + (k & 1) * plane;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        uint64_t D0 = dv[r].x, D1 = dv[r].y;
+        const uint64_t nxp = S1p[(row * Lh + ip) * R + r];
+        const uint64_t nxm = S1p[(row * Lh + im) * R + r];
+        const uint64_t nyp = yh && row + 1 == a.ny ? Hv[r] : S1p[(rp * Lh + i) * R + r];
+        const uint64_t nym = yh && row == 0 ? Hv[r] : S1p[(rm * Lh + i) * R + r];
+        E0[r] = msa::site_update(S0v[r], nxp, nyp, Sp[r], nxm, nym, Sm[r], j0x, j0y, j0z, c0J3,
+                                 c0J4, j1z_prev, D0, D1);
         Ew[slot + r] = E0[r];
         if (write) {
-          a.sp_out[(uint64_t)(2 * r) * H + h] = E0[r];
-          a.dm_out[r * rstride + lbase0 + hw] = make_ulonglong2(D0, D1);
+          sto_stream(a.sp_out + (uint64_t)(2 * r) * H, h * 8u, E0[r]);
+          sto_stream(Dout0 + r * rstride, hw * 16u, D0, D1);
         }
       }
     }
     __syncthreads();
     // ---- colour 1, plane s-1
     if (own && k >= 2) {
-      const uint32_t zo = msa::wrapi(s - 1, L);
-      const uint64_t zob = LhL * zo;
-      const uint32_t off = (y + zo + 1) & 1u;
-      const uint32_t ip = off ? msa::wrapi((int)i + 1, Lh) : i, im = off ? i : msa::wrapi((int)i - 1, Lh);
-      const uint64_t ho = i + ybase + zob;
-      const uint64_t J3 = J[im + ybase + zob], J4 = J[H + i + ymb + zob];
-      const uint64_t hw = msa::walk_dest(g, 1, dx, dy, dz, i, y, zo);
-      const uint64_t *Er = Eb + (uint64_t)((k - 1) & 1) * plane;
+      const uint64_t c1J3 = ldo(Jx0, (im + yb + LhL * zo) * 8u);
+      const uint64_t c1J4 = ldo(Jy0, (i + ymb + LhL * zo) * 8u);
+      ulonglong2 dv[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) dv[r] = ldo_stream(Din1 + r * rstride, ho * 16u);
+      const uint32_t hw = (uint32_t)msa::walk_dest(g, 1, dx, dy, dz, i, y, zo);
+      const uint64_t *Er = Eb + ((k - 1) & 1) * plane;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        const ulonglong2 d = a.dm_in[r * rstride + lbase1 + ho];
-        uint64_t D0 = d.x, D1 = d.y;
-        const uint64_t nxp = Er[((uint64_t)row * Lh + ip) * R + r];
-        const uint64_t nxm = Er[((uint64_t)row * Lh + im) * R + r];
-        const uint64_t nyp = Er[((uint64_t)rp * Lh + i) * R + r];
-        const uint64_t nym = Er[((uint64_t)rm * Lh + i) * R + r];
+        uint64_t D0 = dv[r].x, D1 = dv[r].y;
+        const uint64_t nxp = Er[(row * Lh + ip) * R + r];
+        const uint64_t nxm = Er[(row * Lh + im) * R + r];
+        const uint64_t nyp = Er[(rp * Lh + i) * R + r];
+        const uint64_t nym = Er[(rm * Lh + i) * R + r];
         const uint64_t sn = msa::site_update(Sm[r], nxp, nyp, E0[r], nxm, nym, Em2[r], j1x_prev,
-                                             j1y_prev, j1z_prev, J3, J4, j0z_m2, D0, D1);
-        a.sp_out[(uint64_t)(2 * r + 1) * H + ho] = sn;
-        a.dm_out[r * rstride + lbase1 + hw] = make_ulonglong2(D0, D1);
+                                             j1y_prev, j1z_prev, c1J3, c1J4, j0z_m2, D0, D1);
+        sto_stream(a.sp_out + (uint64_t)(2 * r + 1) * H, ho * 8u, sn);
+        sto_stream(Dout1 + r * rstride, hw * 16u, D0, D1);
       }
     }
     // ---- rotate the windows
@@ -526,6 +542,12 @@ static int fused_tile_ok(const Geo &g, uint32_t ty, uint32_t tz) {
   return 1;
 }
 
+// The second one-pass kernel addresses every buffer with 32-bit byte offsets.
+static bool narrow_ok(const Geo &g) {
+  const uint64_t lim = 1ull << 32;
+  return 16ull * g.R * g.nc * g.V < lim && 24ull * g.V < lim && 8ull * g.R * g.V < lim;
+}
+
 static void set_fused(msa_ctx *c, int mode, uint32_t ty, uint32_t tz) {
   c->fused = mode;
   c->fty = ty;
@@ -551,7 +573,7 @@ static void pick_fused(msa_ctx *c) {
       const double score = fill / waste;
       if (score > best) { best = score; best_ty = ty; best_tz = tz; }
     }
-  if (best_ty) set_fused(c, 2, best_ty, best_tz);
+  if (best_ty) set_fused(c, narrow_ok(g) ? 2 : 1, best_ty, best_tz);
 }
 
 extern "C" int msa_create(int device, const msa_geom *geom, msa_ctx **out) {
@@ -626,6 +648,9 @@ extern "C" int msa_set_tiling(msa_ctx *c, int32_t mode, int32_t ty, int32_t tz) 
   }
   if (mode != 1 && mode != 2)
     return msa_fail(MSA_ERR_ARG, "mode=%d (0 two-pass, 1 and 2 one-pass)", mode);
+  if (mode == 2 && !narrow_ok(c->g))
+    return msa_fail(MSA_ERR_ARG, "mode 2 needs every buffer below 4 GiB (L=%u R=%u nc=%u)", c->g.L,
+                    c->g.R, c->g.nc);
   if (ty == 0 && tz == 0) {
     pick_fused(c);
     if (!c->fused) return msa_fail(MSA_ERR_ARG, "no one-pass tile for L=%u R=%u", c->g.L, c->g.R);
@@ -1016,7 +1041,8 @@ extern "C" int msa_autotune(msa_ctx *c, int32_t sweeps, int32_t *mode, int32_t *
   for (uint32_t y = 1; y <= c->g.L; ++y)
     for (uint32_t z = 2; z <= c->g.L && z <= 32; ++z)
       if (fused_tile_ok(c->g, y, z))
-        for (int32_t m = 1; m <= 2; ++m) cands.push_back({m, (int32_t)y, (int32_t)z});
+        for (int32_t m = 1; m <= (narrow_ok(c->g) ? 2 : 1); ++m)
+          cands.push_back({m, (int32_t)y, (int32_t)z});
   msa_ctx *t = nullptr;
   int rc = msa_create(c->device, &c->geom, &t);
   if (rc) return rc;
