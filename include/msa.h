@@ -12,7 +12,8 @@
  *   J     [3][V]        bond from s to s+e_d, d = x,y,z (one bit per sample)
  *   spin  [R][V]        R replicas share J
  *   d0,d1 [R][nc][V]    the two bit planes of the 2-bit demons (D = d0 + 2 d1)
- * On the device the same words are stored colour-split.
+ * On the device the same words are stored colour-split: the sites of colour
+ * c = (x+y+z)&1 packed as h = x/2 + (L/2)(y + L z), one plane per colour.
  *
  * Every call returns MSA_OK (0) or a negative msa_status; msa_strerror()
  * describes the last error of the calling thread.  A context is not
@@ -175,7 +176,7 @@ int64_t msa_batch_bytes_in_use(msa_batch *b);
 /* Replace the constructed state of a lattice that has not run yet. */
 int msa_batch_upload_state(msa_batch *b, int32_t slot, const uint64_t *J, const uint64_t *spin,
                            const uint64_t *d0, const uint64_t *d1);
-/* Launch shape: threads per block (128, 256, 512) and iterations per graph. */
+/* Launch shape: threads per block (128, 256, 512, 1024) and iterations per graph. */
 int msa_batch_set_shape(msa_batch *b, int32_t block, int32_t chunk);
 /* Advance every unfinished lattice by up to niter iterations (asynchronous). */
 int msa_batch_run(msa_batch *b, int64_t niter);

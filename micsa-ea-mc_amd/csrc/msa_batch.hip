@@ -24,24 +24,36 @@ struct BLat {
   msa::Geo g;
   const uint4 *ctr;
   const uint32_t *key;
-  const msa_lut *luts;
+  const msa::FineLut *luts;
   const int32_t *lut_idx;
   int64_t every;
   uint64_t t, tend;
-  uint32_t ntiles, done;
-  int32_t nref, pad;
+  uint32_t nitems, done;  // work items of this lattice; blocks finished this iteration
+  int32_t nref;
+  uint32_t bx, nr, nxc;   // a tile is bx site pairs of nr rows (y, z); nxc tiles per row block
+  uint32_t mLh, mL;       // reciprocals for n / Lh and n / L (div_recip)
 };
 
+// A block's work: tiles tile0 .. tile0+ntile-1 of lattice `lat`.
 struct BItem {
-  uint32_t lat, tile;
+  uint32_t lat, tile0, ntile, pad;
 };
 
-// One colour of one sweep for every (lattice, tile) item: the update of
-// k_halfsweep (d_oneMCstepBN_multisample + fused walk + fused refresh) for
-// nc = 1, with the lattice's own geometry, buffers, counters and tables.
+// floor(n / d) for n * d < 2^32 with m = floor(2^32 / d) + 1: the error of
+// n * m / 2^32 over n / d is below n / 2^32 < 1 / d, so the floor is exact.
+static inline uint32_t div_recip_of(uint32_t d) { return (uint32_t)((1ull << 32) / d + 1); }
+__device__ __forceinline__ uint32_t div_recip(uint32_t n, uint32_t m) { return __umulhi(n, m); }
+
+// One colour of one sweep for every work item: the update of k_halfsweep
+// (d_oneMCstepBN_multisample + fused walk + fused refresh) for nc = 1, with
+// the lattice's own geometry, buffers, counters and tables.  A block stages
+// the lattice's two search tables in LDS once (when its iteration refreshes)
+// and then updates its tiles.  Index arithmetic is 32-bit (every batched
+// lattice has L^3 < 2^31, checked on admission) and divisions by L/2 and L
+// use per-lattice reciprocals.
 template <int BS>
 __global__ void __launch_bounds__(BS) k_batch_halfsweep(BLat *lats, const BItem *items, int colour) {
-  __shared__ uint32_t s_lut[2 * 1024];
+  extern __shared__ uint4 s_tab[];
   const BItem it = items[blockIdx.x];
   BLat *lp = lats + it.lat;
   const uint64_t t = lp->t;
@@ -49,68 +61,73 @@ __global__ void __launch_bounds__(BS) k_batch_halfsweep(BLat *lats, const BItem 
   const msa::Geo g = lp->g;
   const int64_t k = refresh_index(t, lp->every);
   const bool refresh = k >= 0 && k < lp->nref;
+  const msa::FineLut *tabs = reinterpret_cast<const msa::FineLut *>(s_tab);
   if (refresh) {
-    const uint32_t *src = reinterpret_cast<const uint32_t *>(lp->luts + 2 * (size_t)lp->lut_idx[k]);
-    for (uint32_t j = threadIdx.x; j < 2048; j += BS) s_lut[j] = src[j];
+    const uint4 *src = reinterpret_cast<const uint4 *>(lp->luts + 2 * (size_t)lp->lut_idx[k]);
+    const uint32_t n16 = 2 * sizeof(msa::FineLut) / 16;
+    for (uint32_t j = threadIdx.x; j < n16; j += BS) s_tab[j] = src[j];
     __syncthreads();
   }
+  const msa::LutView t0 = msa::lut_view(tabs), t1 = msa::lut_view(tabs + 1);
   const int par = (int)(t & 1u);
   const uint64_t *A = lp->sp[par];
   uint64_t *B = lp->sp[par ^ 1];
   const ulonglong2 *dmi = lp->dm[par];
   ulonglong2 *dmo = lp->dm[par ^ 1];
-  const uint64_t h = (uint64_t)it.tile * BS + threadIdx.x;
-  if (h < g.H) {
-    const int col = colour;
-    const uint32_t i = (uint32_t)(h % g.Lh);
-    const uint64_t yz = h / g.Lh;
-    const uint32_t y = (uint32_t)(yz % g.L), z = (uint32_t)(yz / g.L);
+  const uint32_t bx = lp->bx, nr = lp->nr, nxc = lp->nxc, Lh = g.Lh, L = g.L;
+  const uint32_t rloc = bx == Lh ? div_recip(threadIdx.x, lp->mLh) : threadIdx.x / bx;
+  const uint32_t xin = threadIdx.x - rloc * bx;
+  const int col = colour;
+  const uint32_t H = (uint32_t)g.H;
+  const uint64_t *Jc = lp->jp + (uint64_t)col * 3 * g.H;
+  const uint64_t *Jo = lp->jp + (uint64_t)(col ^ 1) * 3 * g.H;
+  const uint64_t *Sn = col ? B : A;
+  for (uint32_t tt = 0; tt < it.ntile; ++tt) {
+    const uint32_t tile = it.tile0 + tt;
+    const uint32_t rbk = nxc == 1 ? tile : tile / nxc;
+    const uint32_t i = (tile - rbk * nxc) * bx + xin;
+    const uint32_t row = rbk * nr + rloc;
+    if (rloc >= nr || i >= Lh || row >= L * L) continue;
+    const uint32_t z = div_recip(row, lp->mL), y = row - z * L;
     const uint32_t off = (y + z + (uint32_t)col) & 1u;
     const uint32_t x = 2u * i + off;
-    const uint32_t ip = off ? msa::wrapi((int)i + 1, g.Lh) : i;
-    const uint32_t im = off ? i : msa::wrapi((int)i - 1, g.Lh);
-    const uint32_t yp = msa::wrapi((int)y + 1, g.L), ym = msa::wrapi((int)y - 1, g.L);
-    const uint32_t zp = msa::wrapi((int)z + 1, g.L), zm = msa::wrapi((int)z - 1, g.L);
-    const uint64_t rb = (uint64_t)g.Lh * (y + (uint64_t)g.L * z);
-    const uint64_t hxp = ip + rb, hxm = im + rb;
-    const uint64_t hyp = i + (uint64_t)g.Lh * (yp + (uint64_t)g.L * z);
-    const uint64_t hym = i + (uint64_t)g.Lh * (ym + (uint64_t)g.L * z);
-    const uint64_t hzp = i + (uint64_t)g.Lh * (y + (uint64_t)g.L * zp);
-    const uint64_t hzm = i + (uint64_t)g.Lh * (y + (uint64_t)g.L * zm);
-    const uint64_t *Jc = lp->jp + (uint64_t)col * 3 * g.H;
-    const uint64_t *Jo = lp->jp + (uint64_t)(col ^ 1) * 3 * g.H;
-    const uint64_t J0 = Jc[h], J1 = Jc[g.H + h], J2 = Jc[2 * g.H + h];
-    const uint64_t J3 = Jo[hxm], J4 = Jo[g.H + hym], J5 = Jo[2 * g.H + hzm];
-    const uint64_t hw = msa::walk_dest(g, col, 1, 0, 0, i, y, z);   // layer 0 walks along +x
-    const uint64_t *Sn = col ? B : A;
-    msa::LutView t0{reinterpret_cast<const uint2 *>(s_lut),
-                    reinterpret_cast<const uint16_t *>(s_lut + 512)};
-    msa::LutView t1{reinterpret_cast<const uint2 *>(s_lut + 1024),
-                    reinterpret_cast<const uint16_t *>(s_lut + 1536)};
-    const uint32_t tid = (uint32_t)(x + (uint64_t)g.L * (y + (uint64_t)g.L * z));
+    const uint32_t ip = off ? msa::wrapi((int)i + 1, Lh) : i;
+    const uint32_t im = off ? i : msa::wrapi((int)i - 1, Lh);
+    const uint32_t yp = msa::wrapi((int)y + 1, L), ym = msa::wrapi((int)y - 1, L);
+    const uint32_t zp = msa::wrapi((int)z + 1, L), zm = msa::wrapi((int)z - 1, L);
+    const uint32_t rb = Lh * row;
+    const uint32_t h = i + rb;
+    const uint32_t hxp = ip + rb, hxm = im + rb;
+    const uint32_t hyp = i + Lh * (yp + L * z), hym = i + Lh * (ym + L * z);
+    const uint32_t hzp = i + Lh * (y + L * zp), hzm = i + Lh * (y + L * zm);
+    const uint64_t J0 = Jc[h], J1 = Jc[H + h], J2 = Jc[2 * H + h];
+    const uint64_t J3 = Jo[hxm], J4 = Jo[H + hym], J5 = Jo[2 * H + hzm];
+    const uint32_t hw = (uint32_t)msa::walk_dest(g, col, 1, 0, 0, i, y, z);   // layer 0 walks along +x
+    const uint32_t tid = x + L * row;
     for (uint32_t r = 0; r < g.R; ++r) {
       const uint64_t *So = Sn + (uint64_t)(2 * r + (col ^ 1)) * g.H;
       const uint64_t sown = A[(uint64_t)(2 * r + col) * g.H + h];
-      const uint64_t dbase = ((uint64_t)r * 2 + col) * g.H;
+      const ulonglong2 *Di = dmi + ((uint64_t)r * 2 + col) * g.H;
+      ulonglong2 *Do = dmo + ((uint64_t)r * 2 + col) * g.H;
       uint64_t D0, D1;
       if (refresh) {
         msa::refresh_demon(t0, t1, lp->ctr[(size_t)k * g.R + r], lp->key[r], tid, D0, D1);
       } else {
-        const ulonglong2 d = dmi[dbase + h];
+        const ulonglong2 d = Di[h];
         D0 = d.x;
         D1 = d.y;
       }
       const uint64_t sn = msa::site_update(sown, So[hxp], So[hyp], So[hzp], So[hxm], So[hym],
                                            So[hzm], J0, J1, J2, J3, J4, J5, D0, D1);
       B[(uint64_t)(2 * r + col) * g.H + h] = sn;
-      dmo[dbase + hw] = make_ulonglong2(D0, D1);
+      Do[hw] = make_ulonglong2(D0, D1);
     }
   }
   if (colour == 1) {
     __syncthreads();
     if (threadIdx.x == 0) {
       __threadfence();
-      if (atomicAdd(&lp->done, 1u) == lp->ntiles - 1) {   // last block of this lattice
+      if (atomicAdd(&lp->done, 1u) == lp->nitems - 1) {   // last block of this lattice
         lp->done = 0;
         lp->t = t + 1;
         __threadfence();
@@ -118,6 +135,8 @@ __global__ void __launch_bounds__(BS) k_batch_halfsweep(BLat *lats, const BItem 
     }
   }
 }
+
+static int batch_lds_attr();
 
 struct msa_batch {
   int device = 0;
@@ -132,7 +151,7 @@ struct msa_batch {
   size_t cap = 0;
   BItem *d_items = nullptr;
   uint32_t nitems = 0, items_cap = 0;
-  msa_lut *luts = nullptr;
+  msa::FineLut *luts = nullptr;         // shared search tables, 2 per pair
   int32_t nlut = 0;
   int block = 256;
   int32_t chunk = 16;                   // iterations per captured graph
@@ -171,6 +190,8 @@ extern "C" int msa_batch_create(int device, int64_t max_bytes, msa_batch **out) 
   HIPCHK(hipGetDeviceCount(&ndev));
   if (device < 0 || device >= ndev) return msa_fail(MSA_ERR_ARG, "device=%d count=%d", device, ndev);
   HIPCHK(hipSetDevice(device));
+  int rc = batch_lds_attr();
+  if (rc) return rc;
   msa_batch *b = new msa_batch();
   b->device = device;
   b->max_bytes = max_bytes > 0 ? max_bytes : INT64_MAX;
@@ -208,11 +229,9 @@ extern "C" int msa_batch_set_luts(msa_batch *b, int32_t nlut, const msa_lut *lut
   HIPCHK(hipSetDevice(b->device));
   (void)hipFree(b->luts);
   b->luts = nullptr;
-  if (hipMalloc(&b->luts, sizeof(msa_lut) * 2 * (size_t)nlut) != hipSuccess)
-    return msa_fail(MSA_ERR_NOMEM, "LUT table of %d pairs", nlut);
-  HIPCHK(hipMemcpyAsync(b->luts, luts, sizeof(msa_lut) * 2 * (size_t)nlut, hipMemcpyHostToDevice,
-                        b->stream));
-  HIPCHK(hipStreamSynchronize(b->stream));
+  b->nlut = 0;
+  const int rc = msa_build_fine_luts(b->stream, nlut, luts, &b->luts);
+  if (rc) return rc;
   b->nlut = nlut;
   return MSA_OK;
 }
@@ -229,6 +248,8 @@ extern "C" int64_t msa_batch_job_bytes(const msa_job *job) {
 extern "C" int msa_batch_add(msa_batch *b, const msa_job *j, int32_t *slot_out) {
   if (!b || !j || !slot_out) return msa_fail(MSA_ERR_ARG, "NULL argument");
   if (j->geom.nc != 1) return msa_fail(MSA_ERR_ARG, "batched lattices use one demon layer (nc=%d)", j->geom.nc);
+  if ((uint64_t)j->geom.L * j->geom.L * j->geom.L >= (1ull << 31))
+    return msa_fail(MSA_ERR_ARG, "batched lattices need L^3 < 2^31 (L=%d)", j->geom.L);
   if (!b->luts) return msa_fail(MSA_ERR_STATE, "set the shared LUTs first");
   if (j->niter < 0) return msa_fail(MSA_ERR_ARG, "niter=%lld", (long long)j->niter);
   if (j->refresh_every < -1) return msa_fail(MSA_ERR_ARG, "refresh_every=%lld", (long long)j->refresh_every);
@@ -320,6 +341,28 @@ extern "C" int msa_batch_upload_state(msa_batch *b, int32_t slot, const uint64_t
   return msa_upload_state(b->slot[slot], J, spin, d0, d1);
 }
 
+static void batch_launch(msa_batch *b, int col) {
+  const dim3 grid(b->nitems);
+  const size_t lds = kRefreshLds;
+  switch (b->block) {
+    case 128: hipLaunchKernelGGL(k_batch_halfsweep<128>, grid, dim3(128), lds, b->stream, b->d_lats, b->d_items, col); break;
+    case 512: hipLaunchKernelGGL(k_batch_halfsweep<512>, grid, dim3(512), lds, b->stream, b->d_lats, b->d_items, col); break;
+    case 1024: hipLaunchKernelGGL(k_batch_halfsweep<1024>, grid, dim3(1024), lds, b->stream, b->d_lats, b->d_items, col); break;
+    default: hipLaunchKernelGGL(k_batch_halfsweep<256>, grid, dim3(256), lds, b->stream, b->d_lats, b->d_items, col); break;
+  }
+}
+
+// The batch kernels stage two search tables (68 KiB) in dynamic LDS.
+static int batch_lds_attr() {
+  const void *fns[] = {reinterpret_cast<const void *>(&k_batch_halfsweep<128>),
+                       reinterpret_cast<const void *>(&k_batch_halfsweep<256>),
+                       reinterpret_cast<const void *>(&k_batch_halfsweep<512>),
+                       reinterpret_cast<const void *>(&k_batch_halfsweep<1024>)};
+  for (const void *f : fns)
+    HIPCHK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kRefreshLds));
+  return MSA_OK;
+}
+
 // Push descriptors and the work list; (re)capture the graph.
 static int batch_prepare(msa_batch *b) {
   if (!b->dirty) return MSA_OK;
@@ -330,10 +373,22 @@ static int batch_prepare(msa_batch *b) {
   for (size_t s = 0; s < n; ++s) {
     if (!b->slot[s]) continue;
     BLat &L = b->h_lats[s];
-    L.ntiles = (uint32_t)((L.g.H + b->block - 1) / b->block);
+    const uint32_t Lh = L.g.Lh, rows = L.g.L * L.g.L, bs = (uint32_t)b->block;
+    L.bx = Lh <= bs ? Lh : bs;
+    L.nr = Lh <= bs ? bs / Lh : 1;
+    L.nxc = (Lh + L.bx - 1) / L.bx;
+    L.mLh = div_recip_of(Lh);
+    L.mL = div_recip_of(L.g.L);
     L.done = 0;
-    if (L.t >= L.tend) continue;
-    for (uint32_t k = 0; k < L.ntiles; ++k) items.push_back(BItem{(uint32_t)s, k});
+    // tiles per item: up to 16, so a block stages the search tables once for
+    // several tiles of a large lattice while small lattices keep one tile each
+    const uint32_t ntiles = ((rows + L.nr - 1) / L.nr) * L.nxc;
+    uint32_t per = ntiles / 16;
+    per = per < 1 ? 1 : (per > 16 ? 16 : per);
+    uint32_t ni = 0;
+    for (uint32_t t0 = 0; t0 < ntiles; t0 += per, ++ni)
+      if (L.t < L.tend) items.push_back(BItem{(uint32_t)s, t0, ntiles - t0 < per ? ntiles - t0 : per, 0});
+    L.nitems = ni;
   }
   if (n > b->cap) {
     (void)hipFree(b->d_lats);
@@ -362,12 +417,7 @@ static int batch_prepare(msa_batch *b) {
     HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     for (int32_t it = 0; it < b->chunk; ++it)
       for (int col = 0; col < 2; ++col) {
-        if (b->block == 128)
-          hipLaunchKernelGGL(k_batch_halfsweep<128>, dim3(b->nitems), dim3(128), 0, b->stream, b->d_lats, b->d_items, col);
-        else if (b->block == 512)
-          hipLaunchKernelGGL(k_batch_halfsweep<512>, dim3(b->nitems), dim3(512), 0, b->stream, b->d_lats, b->d_items, col);
-        else
-          hipLaunchKernelGGL(k_batch_halfsweep<256>, dim3(b->nitems), dim3(256), 0, b->stream, b->d_lats, b->d_items, col);
+        batch_launch(b, col);
       }
     HIPCHK(hipStreamEndCapture(b->stream, &b->graph));
     HIPCHK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
@@ -378,7 +428,8 @@ static int batch_prepare(msa_batch *b) {
 
 extern "C" int msa_batch_set_shape(msa_batch *b, int32_t block, int32_t chunk) {
   if (!b) return msa_fail(MSA_ERR_ARG, "NULL batch");
-  if (block != 128 && block != 256 && block != 512) return msa_fail(MSA_ERR_ARG, "block=%d (128, 256 or 512)", block);
+  if (block != 128 && block != 256 && block != 512 && block != 1024)
+    return msa_fail(MSA_ERR_ARG, "block=%d (128, 256, 512 or 1024)", block);
   if (chunk < 1 || chunk > 4096) return msa_fail(MSA_ERR_ARG, "chunk=%d", chunk);
   HIPCHK(hipSetDevice(b->device));
   HIPCHK(hipStreamSynchronize(b->stream));
@@ -407,14 +458,7 @@ extern "C" int msa_batch_run(msa_batch *b, int64_t niter) {
     left -= b->chunk;
   }
   for (int64_t it = 0; it < left; ++it)
-    for (int col = 0; col < 2; ++col) {
-      if (b->block == 128)
-        hipLaunchKernelGGL(k_batch_halfsweep<128>, dim3(b->nitems), dim3(128), 0, b->stream, b->d_lats, b->d_items, col);
-      else if (b->block == 512)
-        hipLaunchKernelGGL(k_batch_halfsweep<512>, dim3(b->nitems), dim3(512), 0, b->stream, b->d_lats, b->d_items, col);
-      else
-        hipLaunchKernelGGL(k_batch_halfsweep<256>, dim3(b->nitems), dim3(256), 0, b->stream, b->d_lats, b->d_items, col);
-    }
+    for (int col = 0; col < 2; ++col) batch_launch(b, col);
   HIPCHK(hipGetLastError());
   return MSA_OK;
 }

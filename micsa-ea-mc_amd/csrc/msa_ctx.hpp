@@ -16,7 +16,7 @@ struct msa_ctx {
   ulonglong2 *dm[2] = {nullptr, nullptr}; // demons, ping-pong
   int scur = 0;                           // spin buffer holding the current state
   int cur = 0;                            // demon buffer holding the current state
-  msa_lut *luts = nullptr;
+  msa::FineLut *luts = nullptr;           // [2 * nlut] search tables (plane 0, plane 1)
   bool luts_owned = true;                 // false: a batch's shared table
   int32_t nlut = 0;
   uint4 *ctr = nullptr;
@@ -24,12 +24,19 @@ struct msa_ctx {
   int32_t nref = 0;
   std::vector<int32_t> lut_idx;           // host copy, indexes luts
   int32_t *meas = nullptr;                // [64]
-  dim3 blk{1, 1, 1}, grd{1, 1, 1};        // two-pass kernel launch
+  uint32_t hnr = 1, hgrid = 1;            // two-pass kernel: rows per tile, persistent blocks
   int fused = 0;                          // sweeps without refresh: 0 two-pass, 1/2 one-pass kernel
   uint32_t fty = 0, ftz = 0, fny = 0;     // its tile: rows, planes, thread rows
   int fyhalo = 1;
 };
 
 
-// Point the context at a LUT table it does not own (a batch's shared table).
-int msa_ctx_share_luts(msa_ctx *c, msa_lut *luts, int32_t nlut);
+// Point the context at search tables it does not own (a batch's shared table).
+int msa_ctx_share_luts(msa_ctx *c, msa::FineLut *luts, int32_t nlut);
+
+// Upload nlut LUT pairs and build their search tables (2 * nlut msa::FineLut)
+// in new device memory *out; synchronous on `st`.
+int msa_build_fine_luts(hipStream_t st, int32_t nlut, const msa_lut *host, msa::FineLut **out);
+
+// Dynamic LDS of the refresh kernels (two search tables).
+constexpr size_t kRefreshLds = 2 * sizeof(msa::FineLut);

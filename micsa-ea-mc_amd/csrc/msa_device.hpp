@@ -94,29 +94,42 @@ __device__ __forceinline__ void philox10(uint32_t &v0, uint32_t &v1, uint32_t &v
   }
 }
 
-// One heat-bath table (s_lut_heat_bath) viewed as the reference kernel views
-// it: 256 x uint2 thresholds and 1024 x uint16 presearch entries.
+// One heat-bath table as the device searches it: the 256 cumulative
+// thresholds of s_lut_heat_bath (header.h:113-116) and a presearch of
+// 2^kPreBits buckets of the top bits of R, entry = min | max << 8, the bytes
+// whose intervals hold the first and the last R of the bucket.  The
+// reference's own presearch has 2^10 buckets (NUMBITSPREBUSQUEDAS = 10); this
+// one is built on the device from the same thresholds (k_fine_luts), finer so
+// that almost every search ends in its bucket.  The byte found is the same
+// for every R: the unique n with umbral[n-1] <= R < umbral[n].
+constexpr int kPreBits = 14;
+struct FineLut {
+  uint32_t umb[512];                 // 256 x (msb32, lsb32)
+  uint16_t pre[1u << kPreBits];
+};
+static_assert(sizeof(FineLut) % 16 == 0, "FineLut is copied in 16-byte units");
+
 struct LutView {
   const uint2 *umb;
   const uint16_t *pre;
 };
+__device__ __forceinline__ LutView lut_view(const FineLut *t) {
+  return LutView{reinterpret_cast<const uint2 *>(t->umb), t->pre};
+}
 
 // Byte n with umbral[n-1] <= R < umbral[n], R = (rm:rl): the bisection of
-// main.cu:615-630 seeded by the 10-bit presearch.  The reference first tests
-// R against umbral[254]; the presearch bracket [min, max] of R's 10-bit prefix
-// always contains the answer (including 255, whose bracket ends at 255), so
-// the search starts from the bracket directly -- same byte for every R
-// (checked against the reference bisection on every interval endpoint and
-// bucket edge of the tables in use).
+// main.cu:615-630 started from the presearch bracket [min, max] of R's
+// bucket, which always contains the answer (the thresholds are
+// non-decreasing, checked on upload).  Same byte for every R as the
+// reference's search (checked against it on every interval endpoint).
 __device__ __forceinline__ uint32_t bisect(const LutView &t, uint32_t rm, uint32_t rl) {
   auto below = [&](uint32_t n) {
     const uint2 u = t.umb[n];
     return (rm == u.x) ? (rl < u.y) : (rm < u.x);
   };
-  const uint32_t e = t.pre[rm >> 22];
+  const uint32_t e = t.pre[rm >> (32 - kPreBits)];
   uint32_t lo = e & 255u, hi = e >> 8;
-  // A valid table (min <= max, checked on upload) needs at most 8 halvings;
-  // the bound keeps a malformed table from looping.
+  // at most 8 halvings of a bracket inside 0..255
   for (int it = 0; it < 9 && hi - lo > 1; ++it) {
     const uint32_t mid = (hi + lo) >> 1;
     if (below(mid)) hi = mid; else lo = mid;

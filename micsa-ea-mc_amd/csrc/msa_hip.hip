@@ -37,80 +37,135 @@ struct SweepArgs {
   ulonglong2 *dm_out;
   Geo g;
   int layer, colour;
-  int refresh;            // 1: generate demons instead of reading them
-  const uint4 *ctr;       // [R] counters of this refresh
-  const uint32_t *key;    // [R]
-  const msa_lut *lut;     // [2] plane-0 and plane-1 tables of this refresh
+  uint32_t nr;              // rows (y, z) per tile; a block is nr x L/2 threads
+  uint32_t mLh, mL;         // div_recip reciprocals of L/2 and L
+  const uint4 *ctr;         // refresh: [R] counters of this refresh
+  const uint32_t *key;      // refresh: [R]
+  const msa::FineLut *lut;  // refresh: [2] plane-0 and plane-1 tables of this refresh
 };
+
+// floor(n / d) for n * d < 2^32 with m = floor(2^32 / d) + 1 (div_recip_of):
+// n * m / 2^32 exceeds n / d by less than n / 2^32 < 1 / d, so the floor is exact.
+static inline uint32_t div_recip_of(uint32_t d) { return (uint32_t)((1ull << 32) / d + 1); }
+__device__ __forceinline__ uint32_t div_recip(uint32_t n, uint32_t m) { return __umulhi(n, m); }
 
 // One colour of a sweep with demon layer `layer` (d_oneMCstepBN_multisample,
 // main.cu:847-1080, ISI branch), with the demon walk of that layer fused into
-// the demon store and, when `refresh` is set, the heat-bath refresh of the
-// layer fused into the demon load.  Demons go dm_in -> dm_out and spins
-// sp_own -> sp_out, so neither the walk's permutation nor the update races
-// with a load.  Thread = one site of the colour, all R replicas (J loaded once).
-// Used for sweeps that refresh (VALU-bound) and as the general fallback.
-__global__ void __launch_bounds__(256) k_halfsweep(SweepArgs a) {
-  __shared__ uint32_t s_lut[2 * 1024];
+// the demon store and, with REFRESH, the heat-bath refresh of the layer fused
+// into the demon load.  Demons go dm_in -> dm_out and spins sp_own -> sp_out,
+// so neither the walk's permutation nor the update races with a load.
+// Thread = one site of the colour, all R replicas (J loaded once).  Blocks
+// are persistent: each stages the two search tables of the refresh in LDS
+// once (2 x 34 KiB) and then walks tiles of nr rows.  Used for every sweep
+// with a refresh (VALU-bound: Philox and the table search) and for the
+// two-pass mode.
+template <bool REFRESH>
+__global__ void __launch_bounds__(1024) k_halfsweep(SweepArgs a) {
+  extern __shared__ uint4 s_tab[];
   const Geo g = a.g;
-  if (a.refresh) {
-    const uint32_t *src = reinterpret_cast<const uint32_t *>(a.lut);
-    const uint32_t nthr = blockDim.x * blockDim.y;
-    for (uint32_t k = threadIdx.y * blockDim.x + threadIdx.x; k < 2048; k += nthr) s_lut[k] = src[k];
+  const msa::FineLut *tabs = reinterpret_cast<const msa::FineLut *>(s_tab);
+  if (REFRESH) {
+    const uint4 *src = reinterpret_cast<const uint4 *>(a.lut);
+    const uint32_t n16 = 2 * sizeof(msa::FineLut) / 16;
+    for (uint32_t k = threadIdx.x; k < n16; k += blockDim.x) s_tab[k] = src[k];
     __syncthreads();
   }
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t y = blockIdx.y * blockDim.y + threadIdx.y;
-  const uint32_t z = blockIdx.z;
-  if (i >= g.Lh || y >= g.L) return;
-
+  const msa::LutView t0 = msa::lut_view(tabs), t1 = msa::lut_view(tabs + 1);
+  const uint32_t Lh = g.Lh, L = g.L;
+  const uint32_t rloc = div_recip(threadIdx.x, a.mLh), i = threadIdx.x - rloc * Lh;
+  const uint32_t nrows = L * L, ntiles = (nrows + a.nr - 1) / a.nr;
   const int col = a.colour;
-  const uint32_t off = (y + z + (uint32_t)col) & 1u;
-  const uint32_t x = 2u * i + off;
-  const uint32_t ip = off ? msa::wrapi((int)i + 1, g.Lh) : i;      // site x+1
-  const uint32_t im = off ? i : msa::wrapi((int)i - 1, g.Lh);      // site x-1
-  const uint32_t yp = msa::wrapi((int)y + 1, g.L), ym = msa::wrapi((int)y - 1, g.L);
-  const uint32_t zp = msa::wrapi((int)z + 1, g.L), zm = msa::wrapi((int)z - 1, g.L);
-  const uint64_t row = (uint64_t)g.Lh * (y + (uint64_t)g.L * z);
-  const uint64_t h = i + row;
-  const uint64_t hxp = ip + row, hxm = im + row;
-  const uint64_t hyp = i + (uint64_t)g.Lh * (yp + (uint64_t)g.L * z);
-  const uint64_t hym = i + (uint64_t)g.Lh * (ym + (uint64_t)g.L * z);
-  const uint64_t hzp = i + (uint64_t)g.Lh * (y + (uint64_t)g.L * zp);
-  const uint64_t hzm = i + (uint64_t)g.Lh * (y + (uint64_t)g.L * zm);
-
-  const uint64_t *Jc = a.jp + (uint64_t)col * 3 * g.H;
-  const uint64_t *Jo = a.jp + (uint64_t)(col ^ 1) * 3 * g.H;
-  const uint64_t J0 = Jc[h], J1 = Jc[g.H + h], J2 = Jc[2 * g.H + h];
-  const uint64_t J3 = Jo[hxm], J4 = Jo[g.H + hym], J5 = Jo[2 * g.H + hzm];
-
   int dx, dy, dz;
   msa::layer_dir(a.layer, dx, dy, dz);
-  const uint64_t hw = msa::walk_dest(g, col, dx, dy, dz, i, y, z);
-
-  msa::LutView t0{reinterpret_cast<const uint2 *>(s_lut),
-                  reinterpret_cast<const uint16_t *>(s_lut + 512)};
-  msa::LutView t1{reinterpret_cast<const uint2 *>(s_lut + 1024),
-                  reinterpret_cast<const uint16_t *>(s_lut + 1536)};
-  const uint32_t tid = (uint32_t)((uint64_t)a.layer * g.V + x + (uint64_t)g.L * (y + (uint64_t)g.L * z));
-
-  for (uint32_t r = 0; r < g.R; ++r) {
-    const uint64_t *So = a.sp_nb + (uint64_t)(2 * r + (col ^ 1)) * g.H;
-    const uint64_t sown = a.sp_own[(uint64_t)(2 * r + col) * g.H + h];
-    const uint64_t dbase = ((uint64_t)(r * g.nc + a.layer) * 2 + col) * g.H;
-    uint64_t D0, D1;
-    if (a.refresh) {
-      msa::refresh_demon(t0, t1, a.ctr[r], a.key[r], tid, D0, D1);
-    } else {
-      const ulonglong2 d = a.dm_in[dbase + h];
-      D0 = d.x;
-      D1 = d.y;
+  const uint64_t *Jc = a.jp + (uint64_t)col * 3 * g.H;
+  const uint64_t *Jo = a.jp + (uint64_t)(col ^ 1) * 3 * g.H;
+  for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const uint32_t row = tile * a.nr + rloc;
+    if (rloc >= a.nr || row >= nrows) continue;
+    const uint32_t z = div_recip(row, a.mL), y = row - z * L;
+    const uint32_t off = (y + z + (uint32_t)col) & 1u;
+    const uint32_t x = 2u * i + off;
+    const uint32_t ip = off ? msa::wrapi((int)i + 1, Lh) : i;      // site x+1
+    const uint32_t im = off ? i : msa::wrapi((int)i - 1, Lh);      // site x-1
+    const uint32_t yp = msa::wrapi((int)y + 1, L), ym = msa::wrapi((int)y - 1, L);
+    const uint32_t zp = msa::wrapi((int)z + 1, L), zm = msa::wrapi((int)z - 1, L);
+    const uint32_t H = (uint32_t)g.H;
+    const uint32_t rb = Lh * row, h = i + rb, hxp = ip + rb, hxm = im + rb;
+    const uint32_t hyp = i + Lh * (yp + L * z), hym = i + Lh * (ym + L * z);
+    const uint32_t hzp = i + Lh * (y + L * zp), hzm = i + Lh * (y + L * zm);
+    const uint64_t J0 = Jc[h], J1 = Jc[H + h], J2 = Jc[2 * H + h];
+    const uint64_t J3 = Jo[hxm], J4 = Jo[H + hym], J5 = Jo[2 * H + hzm];
+    const uint64_t hw = msa::walk_dest(g, col, dx, dy, dz, i, y, z);
+    const uint32_t tid = (uint32_t)((uint64_t)a.layer * g.V + x + (uint64_t)L * row);
+    for (uint32_t r = 0; r < g.R; ++r) {
+      const uint64_t *So = a.sp_nb + (uint64_t)(2 * r + (col ^ 1)) * g.H;
+      const uint64_t sown = a.sp_own[(uint64_t)(2 * r + col) * g.H + h];
+      const uint64_t dbase = ((uint64_t)(r * g.nc + a.layer) * 2 + col) * g.H;
+      uint64_t D0, D1;
+      if (REFRESH) {
+        msa::refresh_demon(t0, t1, a.ctr[r], a.key[r], tid, D0, D1);
+      } else {
+        const ulonglong2 d = a.dm_in[dbase + h];
+        D0 = d.x;
+        D1 = d.y;
+      }
+      const uint64_t sn = msa::site_update(sown, So[hxp], So[hyp], So[hzp], So[hxm], So[hym],
+                                           So[hzm], J0, J1, J2, J3, J4, J5, D0, D1);
+      a.sp_out[(uint64_t)(2 * r + col) * g.H + h] = sn;
+      a.dm_out[dbase + hw] = make_ulonglong2(D0, D1);
     }
-    const uint64_t sn = msa::site_update(sown, So[hxp], So[hyp], So[hzp], So[hxm], So[hym], So[hzm],
-                                         J0, J1, J2, J3, J4, J5, D0, D1);
-    a.sp_out[(uint64_t)(2 * r + col) * g.H + h] = sn;
-    a.dm_out[dbase + hw] = make_ulonglong2(D0, D1);
   }
+}
+
+// Search tables of `ntab` heat-bath tables (msa::FineLut): thresholds copied,
+// presearch bucket b = the bytes of its first and last R, found by binary
+// search for the smallest n with R < umbral[n] (255 when none).
+__global__ void __launch_bounds__(256) k_fine_luts(const msa_lut *luts, uint32_t ntab,
+                                                   msa::FineLut *out) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t tab = blockIdx.y;
+  if (b >= (1u << msa::kPreBits) || tab >= ntab) return;
+  const uint32_t *u = luts[tab].umbrales;
+  auto byte_of = [&](uint64_t R) {
+    uint32_t lo = 0, hi = 255;
+    while (lo < hi) {
+      const uint32_t m = (lo + hi) >> 1;
+      const uint64_t t = ((uint64_t)u[2 * m] << 32) | u[2 * m + 1];
+      if (R < t) hi = m; else lo = m + 1;
+    }
+    return lo;
+  };
+  const uint64_t r0 = (uint64_t)b << (64 - msa::kPreBits);
+  const uint64_t r1 = r0 | ((1ull << (64 - msa::kPreBits)) - 1);
+  out[tab].pre[b] = (uint16_t)(byte_of(r0) | (byte_of(r1) << 8));
+  if (b < 512) out[tab].umb[b] = u[b];
+}
+
+int msa_build_fine_luts(hipStream_t st, int32_t nlut, const msa_lut *host, msa::FineLut **out) {
+  *out = nullptr;
+  msa_lut *raw = nullptr;
+  const size_t ntab = 2 * (size_t)nlut;
+  if (hipMalloc(&raw, sizeof(msa_lut) * ntab) != hipSuccess)
+    return msa_fail(MSA_ERR_NOMEM, "LUT upload of %d pairs", nlut);
+  if (hipMalloc(out, sizeof(msa::FineLut) * ntab) != hipSuccess) {
+    (void)hipFree(raw);
+    return msa_fail(MSA_ERR_NOMEM, "search tables of %d pairs", nlut);
+  }
+  int rc = MSA_OK;
+  if (hipMemcpyAsync(raw, host, sizeof(msa_lut) * ntab, hipMemcpyHostToDevice, st) != hipSuccess)
+    rc = msa_fail(MSA_ERR_HIP, "LUT upload");
+  if (!rc) {
+    hipLaunchKernelGGL(k_fine_luts, dim3((1u << msa::kPreBits) / 256, (uint32_t)ntab), dim3(256), 0, st,
+                       raw, (uint32_t)ntab, *out);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
+      rc = msa_fail(MSA_ERR_HIP, "search table construction");
+  }
+  (void)hipFree(raw);
+  if (rc) {
+    (void)hipFree(*out);
+    *out = nullptr;
+  }
+  return rc;
 }
 
 struct FusedArgs {
@@ -512,19 +567,23 @@ extern "C" int64_t msa_footprint_bytes(const msa_geom *geom) {
   return 8 * V * (3 + 2 * geom->R) + 2 * 16 * V * geom->R * geom->nc + 256;
 }
 
-// Block shape for the two-pass kernel: x spans a whole row of one colour when
-// it fits a wavefront, else the largest divisor of L/2 not above 64.
-static void pick_launch(msa_ctx *c) {
-  const uint32_t Lh = c->g.Lh;
-  uint32_t bx = Lh;
-  if (bx > 64)
-    for (bx = 64; bx > 1; --bx)
-      if (Lh % bx == 0) break;
-  uint32_t by = 256 / bx;
-  if (by > c->g.L) by = c->g.L;
-  if (by < 1) by = 1;
-  c->blk = dim3(bx, by, 1);
-  c->grd = dim3((Lh + bx - 1) / bx, (c->g.L + by - 1) / by, c->g.L);
+// Tile of the two-pass kernel: nr rows of L/2 site pairs per block.  Blocks
+// are persistent, one per CU (the refresh kernel's 68 KiB of LDS and its
+// registers leave room for no second large block), so the block is made as
+// large as possible (most waves per CU to hide the latency of the table
+// searches) while every CU still gets a tile, and never below one wavefront.
+static void pick_half(msa_ctx *c) {
+  const uint32_t Lh = c->g.Lh, rows = c->g.L * c->g.L;
+  int ncu = 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || ncu < 1)
+    ncu = 256;
+  const uint32_t nr_max = 1024 / Lh > 0 ? 1024 / Lh : 1;
+  const uint32_t nr_min = (64 + Lh - 1) / Lh < nr_max ? (64 + Lh - 1) / Lh : nr_max;
+  uint32_t nr = nr_max;
+  while (nr > nr_min && (rows + nr - 1) / nr < (uint32_t)ncu) --nr;
+  const uint32_t tiles = (rows + nr - 1) / nr;
+  c->hnr = nr;
+  c->hgrid = tiles < (uint32_t)ncu ? tiles : (uint32_t)ncu;
 }
 
 static bool fused_supported_R(uint32_t R) { return R == 1 || R == 2 || R == 4; }
@@ -614,7 +673,11 @@ extern "C" int msa_create(int device, const msa_geom *geom, msa_ctx **out) {
       hipMemsetAsync(c->jp, 0, 8 * V * 3, c->stream) != hipSuccess ||
       hipStreamSynchronize(c->stream) != hipSuccess)
     return cleanup(msa_fail(MSA_ERR_HIP, "initial clear failed"));
-  pick_launch(c);
+  // the refresh kernel stages two search tables (68 KiB) in LDS
+  if (hipFuncSetAttribute(reinterpret_cast<const void *>(&k_halfsweep<true>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)kRefreshLds) != hipSuccess)
+    return cleanup(msa_fail(MSA_ERR_HIP, "refresh kernel LDS attribute"));
+  pick_half(c);
   pick_fused(c);
   *out = c;
   return MSA_OK;
@@ -787,16 +850,13 @@ extern "C" int msa_set_luts(msa_ctx *c, int32_t nlut, const msa_lut *luts) {
   c->luts = nullptr;
   c->luts_owned = true;
   c->nlut = 0;
-  if (hipMalloc(&c->luts, sizeof(msa_lut) * 2 * (size_t)nlut) != hipSuccess)
-    return msa_fail(MSA_ERR_NOMEM, "LUT table of %d pairs", nlut);
-  HIPCHK(hipMemcpyAsync(c->luts, luts, sizeof(msa_lut) * 2 * (size_t)nlut, hipMemcpyHostToDevice,
-                        c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
+  const int rc = msa_build_fine_luts(c->stream, nlut, luts, &c->luts);
+  if (rc) return rc;
   c->nlut = nlut;
   return MSA_OK;
 }
 
-int msa_ctx_share_luts(msa_ctx *c, msa_lut *luts, int32_t nlut) {
+int msa_ctx_share_luts(msa_ctx *c, msa::FineLut *luts, int32_t nlut) {
   if (c->luts_owned) (void)hipFree(c->luts);
   c->luts = luts;
   c->nlut = nlut;
@@ -886,22 +946,25 @@ static int launch_run(msa_ctx *c, uint64_t t0, int64_t niter, int64_t every) {
         a.dm_out = dmo;
         a.g = c->g;
         a.layer = (int)l;
-        a.refresh = k >= 0;
+        a.nr = c->hnr;
+        a.mLh = div_recip_of(c->g.Lh);
+        a.mL = div_recip_of(c->g.L);
         if (k >= 0) {
           a.ctr = c->ctr + (size_t)k * c->g.R;
           a.key = c->key;
           a.lut = c->luts + 2 * (size_t)c->lut_idx[k];
         }
-        a.colour = 0;
-        a.sp_own = A;
-        a.sp_nb = A;
-        a.sp_out = B;
-        hipLaunchKernelGGL(k_halfsweep, c->grd, c->blk, 0, c->stream, a);
-        a.colour = 1;
-        a.sp_own = A;
-        a.sp_nb = B;
-        a.sp_out = B;
-        hipLaunchKernelGGL(k_halfsweep, c->grd, c->blk, 0, c->stream, a);
+        const dim3 grid(c->hgrid), block(c->hnr * c->g.Lh);
+        for (int col = 0; col < 2; ++col) {
+          a.colour = col;
+          a.sp_own = A;
+          a.sp_nb = col ? B : A;
+          a.sp_out = B;
+          if (k >= 0)
+            hipLaunchKernelGGL(k_halfsweep<true>, grid, block, kRefreshLds, c->stream, a);
+          else
+            hipLaunchKernelGGL(k_halfsweep<false>, grid, block, 0, c->stream, a);
+        }
       }
       c->scur ^= 1;
     }
