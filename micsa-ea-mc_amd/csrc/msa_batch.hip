@@ -100,23 +100,28 @@ __global__ void __launch_bounds__(BS) k_batch_halfsweep(BLat *lats, const BItem 
     const uint32_t hxp = ip + rb, hxm = im + rb;
     const uint32_t hyp = i + Lh * (yp + L * z), hym = i + Lh * (ym + L * z);
     const uint32_t hzp = i + Lh * (y + L * zp), hzm = i + Lh * (y + L * zm);
-    const uint64_t J0 = Jc[h], J1 = Jc[H + h], J2 = Jc[2 * H + h];
-    const uint64_t J3 = Jo[hxm], J4 = Jo[H + hym], J5 = Jo[2 * H + hzm];
     const uint32_t hw = (uint32_t)msa::walk_dest(g, col, 1, 0, 0, i, y, z);   // layer 0 walks along +x
     const uint32_t tid = x + L * row;
+    uint64_t J0, J1, J2, J3, J4, J5;
     for (uint32_t r = 0; r < g.R; ++r) {
       const uint64_t *So = Sn + (uint64_t)(2 * r + (col ^ 1)) * g.H;
-      const uint64_t sown = A[(uint64_t)(2 * r + col) * g.H + h];
       const ulonglong2 *Di = dmi + ((uint64_t)r * 2 + col) * g.H;
       ulonglong2 *Do = dmo + ((uint64_t)r * 2 + col) * g.H;
       uint64_t D0, D1;
       if (refresh) {
+        // the refresh first: its temporaries are dead before J and the
+        // neighbours are loaded
         msa::refresh_demon(t0, t1, lp->ctr[(size_t)k * g.R + r], lp->key[r], tid, D0, D1);
       } else {
         const ulonglong2 d = Di[h];
         D0 = d.x;
         D1 = d.y;
       }
+      if (r == 0) {
+        J0 = Jc[h]; J1 = Jc[H + h]; J2 = Jc[2 * H + h];
+        J3 = Jo[hxm]; J4 = Jo[H + hym]; J5 = Jo[2 * H + hzm];
+      }
+      const uint64_t sown = A[(uint64_t)(2 * r + col) * g.H + h];
       const uint64_t sn = msa::site_update(sown, So[hxp], So[hyp], So[hzp], So[hxm], So[hym],
                                            So[hzm], J0, J1, J2, J3, J4, J5, D0, D1);
       B[(uint64_t)(2 * r + col) * g.H + h] = sn;
@@ -352,7 +357,7 @@ static void batch_launch(msa_batch *b, int col) {
   }
 }
 
-// The batch kernels stage two search tables (68 KiB) in dynamic LDS.
+// The batch kernels stage two search tables (37 KiB) in dynamic LDS.
 static int batch_lds_attr() {
   const void *fns[] = {reinterpret_cast<const void *>(&k_batch_halfsweep<128>),
                        reinterpret_cast<const void *>(&k_batch_halfsweep<256>),

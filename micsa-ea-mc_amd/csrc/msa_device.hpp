@@ -95,40 +95,43 @@ __device__ __forceinline__ void philox10(uint32_t &v0, uint32_t &v1, uint32_t &v
 }
 
 // One heat-bath table as the device searches it: the 256 cumulative
-// thresholds of s_lut_heat_bath (header.h:113-116) and a presearch of
-// 2^kPreBits buckets of the top bits of R, entry = min | max << 8, the bytes
-// whose intervals hold the first and the last R of the bucket.  The
-// reference's own presearch has 2^10 buckets (NUMBITSPREBUSQUEDAS = 10); this
-// one is built on the device from the same thresholds (k_fine_luts), finer so
-// that almost every search ends in its bucket.  The byte found is the same
-// for every R: the unique n with umbral[n-1] <= R < umbral[n].
+// thresholds of s_lut_heat_bath (header.h:113-116) and a presearch over the
+// top kPreBits bits of R: pre[b] = the byte whose interval holds the first R
+// of bucket b, pre[2^kPreBits] = 255.  The byte of any R in bucket b lies in
+// [pre[b], pre[b+1]] because the search answer is non-decreasing in R.  The
+// reference's own presearch has 2^10 buckets of (min, max) pairs
+// (NUMBITSPREBUSQUEDAS = 10); this one is built on the device from the same
+// thresholds (k_fine_luts), finer so that almost every search ends in its
+// bucket, and half the size per bucket so two tables stage into LDS cheaply.
+// The byte found is the same for every R: the unique n with
+// umbral[n-1] <= R < umbral[n].
 constexpr int kPreBits = 14;
 struct FineLut {
-  uint32_t umb[512];                 // 256 x (msb32, lsb32)
-  uint16_t pre[1u << kPreBits];
+  uint32_t umb[512];                       // 256 x (msb32, lsb32)
+  uint8_t pre[(1u << kPreBits) + 16];      // + the sentinel 255, padded
 };
 static_assert(sizeof(FineLut) % 16 == 0, "FineLut is copied in 16-byte units");
 
 struct LutView {
   const uint2 *umb;
-  const uint16_t *pre;
+  const uint8_t *pre;
 };
 __device__ __forceinline__ LutView lut_view(const FineLut *t) {
   return LutView{reinterpret_cast<const uint2 *>(t->umb), t->pre};
 }
 
 // Byte n with umbral[n-1] <= R < umbral[n], R = (rm:rl): the bisection of
-// main.cu:615-630 started from the presearch bracket [min, max] of R's
-// bucket, which always contains the answer (the thresholds are
-// non-decreasing, checked on upload).  Same byte for every R as the
-// reference's search (checked against it on every interval endpoint).
+// main.cu:615-630 started from the bracket [pre[b], pre[b+1]] of R's bucket
+// b, which always contains the answer (the thresholds are non-decreasing,
+// checked on upload).  Same byte for every R as the reference's search
+// (checked against it on every interval endpoint).
 __device__ __forceinline__ uint32_t bisect(const LutView &t, uint32_t rm, uint32_t rl) {
   auto below = [&](uint32_t n) {
     const uint2 u = t.umb[n];
     return (rm == u.x) ? (rl < u.y) : (rm < u.x);
   };
-  const uint32_t e = t.pre[rm >> (32 - kPreBits)];
-  uint32_t lo = e & 255u, hi = e >> 8;
+  const uint32_t b = rm >> (32 - kPreBits);
+  uint32_t lo = t.pre[b], hi = t.pre[b + 1];
   // at most 8 halvings of a bracket inside 0..255
   for (int it = 0; it < 9 && hi - lo > 1; ++it) {
     const uint32_t mid = (hi + lo) >> 1;

@@ -56,7 +56,7 @@ __device__ __forceinline__ uint32_t div_recip(uint32_t n, uint32_t m) { return _
 // so neither the walk's permutation nor the update races with a load.
 // Thread = one site of the colour, all R replicas (J loaded once).  Blocks
 // are persistent: each stages the two search tables of the refresh in LDS
-// once (2 x 34 KiB) and then walks tiles of nr rows.  Used for every sweep
+// once (2 x 18 KiB) and then walks tiles of nr rows.  Used for every sweep
 // with a refresh (VALU-bound: Philox and the table search) and for the
 // two-pass mode.
 template <bool REFRESH>
@@ -93,22 +93,27 @@ __global__ void __launch_bounds__(1024) k_halfsweep(SweepArgs a) {
     const uint32_t rb = Lh * row, h = i + rb, hxp = ip + rb, hxm = im + rb;
     const uint32_t hyp = i + Lh * (yp + L * z), hym = i + Lh * (ym + L * z);
     const uint32_t hzp = i + Lh * (y + L * zp), hzm = i + Lh * (y + L * zm);
-    const uint64_t J0 = Jc[h], J1 = Jc[H + h], J2 = Jc[2 * H + h];
-    const uint64_t J3 = Jo[hxm], J4 = Jo[H + hym], J5 = Jo[2 * H + hzm];
     const uint64_t hw = msa::walk_dest(g, col, dx, dy, dz, i, y, z);
     const uint32_t tid = (uint32_t)((uint64_t)a.layer * g.V + x + (uint64_t)L * row);
+    uint64_t J0, J1, J2, J3, J4, J5;
     for (uint32_t r = 0; r < g.R; ++r) {
       const uint64_t *So = a.sp_nb + (uint64_t)(2 * r + (col ^ 1)) * g.H;
-      const uint64_t sown = a.sp_own[(uint64_t)(2 * r + col) * g.H + h];
       const uint64_t dbase = ((uint64_t)(r * g.nc + a.layer) * 2 + col) * g.H;
       uint64_t D0, D1;
       if (REFRESH) {
+        // the refresh first: its temporaries are dead before J and the
+        // neighbours are loaded
         msa::refresh_demon(t0, t1, a.ctr[r], a.key[r], tid, D0, D1);
       } else {
         const ulonglong2 d = a.dm_in[dbase + h];
         D0 = d.x;
         D1 = d.y;
       }
+      if (r == 0) {
+        J0 = Jc[h]; J1 = Jc[H + h]; J2 = Jc[2 * H + h];
+        J3 = Jo[hxm]; J4 = Jo[H + hym]; J5 = Jo[2 * H + hzm];
+      }
+      const uint64_t sown = a.sp_own[(uint64_t)(2 * r + col) * g.H + h];
       const uint64_t sn = msa::site_update(sown, So[hxp], So[hyp], So[hzp], So[hxm], So[hym],
                                            So[hzm], J0, J1, J2, J3, J4, J5, D0, D1);
       a.sp_out[(uint64_t)(2 * r + col) * g.H + h] = sn;
@@ -118,26 +123,28 @@ __global__ void __launch_bounds__(1024) k_halfsweep(SweepArgs a) {
 }
 
 // Search tables of `ntab` heat-bath tables (msa::FineLut): thresholds copied,
-// presearch bucket b = the bytes of its first and last R, found by binary
-// search for the smallest n with R < umbral[n] (255 when none).
+// presearch entry b = the byte of the first R of bucket b, found by binary
+// search for the smallest n with R < umbral[n] (255 when none), and the
+// sentinel 255 after the last bucket.
 __global__ void __launch_bounds__(256) k_fine_luts(const msa_lut *luts, uint32_t ntab,
                                                    msa::FineLut *out) {
   const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t tab = blockIdx.y;
-  if (b >= (1u << msa::kPreBits) || tab >= ntab) return;
+  constexpr uint32_t nb = 1u << msa::kPreBits;
+  if (b >= nb + 16 || tab >= ntab) return;
+  if (b >= nb) {
+    out[tab].pre[b] = 255;
+    return;
+  }
   const uint32_t *u = luts[tab].umbrales;
-  auto byte_of = [&](uint64_t R) {
-    uint32_t lo = 0, hi = 255;
-    while (lo < hi) {
-      const uint32_t m = (lo + hi) >> 1;
-      const uint64_t t = ((uint64_t)u[2 * m] << 32) | u[2 * m + 1];
-      if (R < t) hi = m; else lo = m + 1;
-    }
-    return lo;
-  };
-  const uint64_t r0 = (uint64_t)b << (64 - msa::kPreBits);
-  const uint64_t r1 = r0 | ((1ull << (64 - msa::kPreBits)) - 1);
-  out[tab].pre[b] = (uint16_t)(byte_of(r0) | (byte_of(r1) << 8));
+  const uint64_t R = (uint64_t)b << (64 - msa::kPreBits);
+  uint32_t lo = 0, hi = 255;
+  while (lo < hi) {
+    const uint32_t m = (lo + hi) >> 1;
+    const uint64_t t = ((uint64_t)u[2 * m] << 32) | u[2 * m + 1];
+    if (R < t) hi = m; else lo = m + 1;
+  }
+  out[tab].pre[b] = (uint8_t)lo;
   if (b < 512) out[tab].umb[b] = u[b];
 }
 
@@ -155,7 +162,7 @@ int msa_build_fine_luts(hipStream_t st, int32_t nlut, const msa_lut *host, msa::
   if (hipMemcpyAsync(raw, host, sizeof(msa_lut) * ntab, hipMemcpyHostToDevice, st) != hipSuccess)
     rc = msa_fail(MSA_ERR_HIP, "LUT upload");
   if (!rc) {
-    hipLaunchKernelGGL(k_fine_luts, dim3((1u << msa::kPreBits) / 256, (uint32_t)ntab), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_fine_luts, dim3((1u << msa::kPreBits) / 256 + 1, (uint32_t)ntab), dim3(256), 0, st,
                        raw, (uint32_t)ntab, *out);
     if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
       rc = msa_fail(MSA_ERR_HIP, "search table construction");
@@ -568,10 +575,10 @@ extern "C" int64_t msa_footprint_bytes(const msa_geom *geom) {
 }
 
 // Tile of the two-pass kernel: nr rows of L/2 site pairs per block.  Blocks
-// are persistent, one per CU (the refresh kernel's 68 KiB of LDS and its
-// registers leave room for no second large block), so the block is made as
-// large as possible (most waves per CU to hide the latency of the table
-// searches) while every CU still gets a tile, and never below one wavefront.
+// are persistent, one per CU (the refresh kernel's registers allow 16 waves
+// per CU, one block of up to 1024 threads), so the block is made as large as
+// possible (most waves per CU to hide the latency of the table searches)
+// while every CU still gets a tile, and never below one wavefront.
 static void pick_half(msa_ctx *c) {
   const uint32_t Lh = c->g.Lh, rows = c->g.L * c->g.L;
   int ncu = 0;
@@ -673,7 +680,7 @@ extern "C" int msa_create(int device, const msa_geom *geom, msa_ctx **out) {
       hipMemsetAsync(c->jp, 0, 8 * V * 3, c->stream) != hipSuccess ||
       hipStreamSynchronize(c->stream) != hipSuccess)
     return cleanup(msa_fail(MSA_ERR_HIP, "initial clear failed"));
-  // the refresh kernel stages two search tables (68 KiB) in LDS
+  // the refresh kernel stages two search tables (37 KiB) in LDS
   if (hipFuncSetAttribute(reinterpret_cast<const void *>(&k_halfsweep<true>),
                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)kRefreshLds) != hipSuccess)
     return cleanup(msa_fail(MSA_ERR_HIP, "refresh kernel LDS attribute"));
