@@ -134,7 +134,29 @@ class Xoshiro256pp:
         return out
 
 
-def batched_vs_plain(pkg, n_lat, len_div, block=256, chunk=64):
+def plain_seconds(pkg, tables, reqs, jobs):
+    """Device time of the plain form: each lattice alone through msa_run, one
+    after the other (its setup not counted), same tables and seeds as the
+    batched and served forms."""
+    total = 0.0
+    for req, job in zip(reqs, jobs):
+        lat = pkg.Lattice(req.L, req.R, 1)
+        lat.init_philox(req.seed)
+        lat.set_luts(tables.luts)
+        ctr = np.ctypeslib.as_array(job.ctr, shape=(job.nref * 4 * req.R,)).copy()
+        idx = np.ctypeslib.as_array(job.lut_idx, shape=(job.nref,)).copy()
+        key = np.ctypeslib.as_array(job.key, shape=(req.R,)).copy()
+        lat.set_refresh(ctr, key, idx)
+        lat.sync()
+        t0 = time.perf_counter()
+        lat.run(0, req.niter, 1)
+        lat.sync()
+        total += time.perf_counter() - t0
+        lat.close()
+    return total
+
+
+def batched_vs_plain(pkg, n_lat, len_div, block=None, chunk=64):
     """The batched configuration scaled down: n_lat lattices, L uniform over
     {16,24,32,48,64,96}, lengths uniform over {4096,16384,65536}/len_div,
     T 2.0->0.7, refresh every sweep, R=1; drawn with xoshiro256++ seeded 4242.
@@ -153,23 +175,11 @@ def batched_vs_plain(pkg, n_lat, len_div, block=256, chunk=64):
         req = serve.Request(i, L, n, 2.0, 0.7, 77 + i, 0.0)
         jobs.append((req, tables.job(req, i)))
     updates = sum(64 * L ** 3 * n for L, n in lats)
-    # plain
-    t_plain = 0.0
-    for req, job in jobs:
-        lat = pkg.Lattice(req.L, 1, 1)
-        lat.init_philox(req.seed)
-        lat.set_luts(tables.luts)
-        ctr = np.ctypeslib.as_array(job.ctr, shape=(job.nref * 4,)).copy()
-        idx = np.ctypeslib.as_array(job.lut_idx, shape=(job.nref,)).copy()
-        key = np.ctypeslib.as_array(job.key, shape=(1,)).copy()
-        lat.set_refresh(ctr, key, idx)
-        lat.sync()
-        t0 = time.perf_counter()
-        lat.run(0, req.niter, 1)
-        lat.sync()
-        t_plain += time.perf_counter() - t0
-        lat.close()
-    # batched
+    t_plain = plain_seconds(pkg, tables, [req for req, _ in jobs], [job for _, job in jobs])
+    # batched, with the block size picked by the autotuner on the same lattices
+    tuned = None
+    if block is None:
+        block, tuned = serve.tune_block(tables, [req for req, _ in jobs], jobs=[j for _, j in jobs])
     b = msa.Batch(tables.luts)
     b.set_shape(block, chunk)
     for req, job in jobs:
@@ -185,6 +195,7 @@ def batched_vs_plain(pkg, n_lat, len_div, block=256, chunk=64):
     assert (t == te).all()
     b.close()
     return {"lattices": n_lat, "length_divisor": len_div, "block": block, "graph_chunk": chunk,
+            "block_tuning_s": {str(k): round(v, 5) for k, v in tuned.items()} if tuned else None,
             "spin_updates": updates,
             "plain_spin_updates_per_ps": round(updates / t_plain / 1e12, 4),
             "batched_spin_updates_per_ps": round(updates / t_batch / 1e12, 4),
@@ -192,16 +203,40 @@ def batched_vs_plain(pkg, n_lat, len_div, block=256, chunk=64):
             "plain_s": round(t_plain, 3), "batched_s": round(t_batch, 3)}
 
 
-def served(pkg, n_req, rate, log2_len, max_bytes, epoch=64):
-    """Served configuration scaled down: Poisson arrivals (seed 5151), L uniform
-    over {16,32,48,64,96,128}, length 2^U{log2_len}, T 2.0 -> U[0.6,0.9],
-    refresh every sweep, admission by device footprint."""
+def served(pkg, n_req, load, log2_len, max_bytes, epoch=64):
+    """Served configuration scaled down: L uniform over {16,32,48,64,96,128},
+    length 2^U{log2_len}, T 2.0 -> U[0.6,0.9], refresh every sweep, admission
+    by device footprint.  The batched capacity is measured first (all requests
+    present at t = 0); the Poisson stream (seed 5151) then arrives at `load`
+    times that capacity.  The plain form serves the same stream one request
+    at a time in arrival order, each alone through msa_run (its finish times
+    computed from each request's measured plain device time)."""
     import importlib
     serve = importlib.import_module("micsa_ea_mc_amd.serve")
+    tables = serve.Tables()
+    burst = serve.make_requests(n_req, 1e9, seed=5151, log2_len=log2_len)
+    _, cap = serve.serve(burst, max_bytes=max_bytes, epoch=epoch, tables=tables)
+    work = [64 * r.R * r.L ** 3 * r.niter for r in burst]
+    capacity = sum(work) / cap["makespan_s"]                      # updates / s
+    rate = load * capacity / (sum(work) / len(work))              # requests / s
     reqs = serve.make_requests(n_req, rate, seed=5151, log2_len=log2_len)
-    _, summ = serve.serve(reqs, max_bytes=max_bytes, epoch=epoch)
-    summ.update({"arrival_rate_per_s": rate, "log2_length": list(log2_len),
-                 "max_bytes": max_bytes})
+    _, summ = serve.serve(reqs, max_bytes=max_bytes, epoch=epoch, tables=tables)
+    # plain: FIFO, one request alone at a time
+    tp = [plain_seconds(pkg, tables, [r], [tables.job(r, r.rid)]) for r in reqs]
+    finish, lat = 0.0, []
+    for r, t in sorted(zip(reqs, tp), key=lambda x: x[0].arrival):
+        finish = max(finish, r.arrival) + t
+        lat.append(finish - r.arrival)
+    updates = sum(64 * r.R * r.L ** 3 * r.niter for r in reqs)
+    summ.update({"load_of_capacity": load, "arrival_rate_per_s": round(rate, 2),
+                 "batched_capacity_spin_updates_per_ps": round(capacity / 1e12, 4),
+                 "log2_length": list(log2_len), "max_bytes": max_bytes,
+                 "plain_fifo_makespan_s": round(finish, 4),
+                 "plain_fifo_spin_updates_per_ps": round(updates / finish / 1e12, 4),
+                 "plain_fifo_latency_p50_s": round(float(np.percentile(lat, 50)), 4),
+                 "plain_fifo_latency_p99_s": round(float(np.percentile(lat, 99)), 4),
+                 "plain_device_spin_updates_per_ps": round(updates / sum(tp) / 1e12, 4),
+                 "served_over_plain": round(finish / summ["makespan_s"], 3)})
     for k in ("makespan_s", "spin_updates_per_ps", "latency_p50_s", "latency_p99_s"):
         summ[k] = round(summ[k], 4)
     return summ
@@ -222,7 +257,8 @@ def main():
     ap.add_argument("--batched", type=int, default=48, help="lattices in the batched test (0: skip)")
     ap.add_argument("--batched-len-div", type=int, default=64)
     ap.add_argument("--served", type=int, default=60, help="requests in the served test (0: skip)")
-    ap.add_argument("--served-rate", type=float, default=40.0)
+    ap.add_argument("--served-load", type=float, default=0.8,
+                    help="arrival rate as a fraction of the measured batched capacity")
     ap.add_argument("--autotune", action="store_true", help="autotune the sweep kernel first")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -311,7 +347,7 @@ def main():
     if args.batched > 0:
         out["batched"] = batched_vs_plain(pkg, args.batched, args.batched_len_div)
     if args.served > 0:
-        out["served"] = served(pkg, args.served, args.served_rate, (6, 11), 8 << 30)
+        out["served"] = served(pkg, args.served, args.served_load, (6, 11), 8 << 30)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(L, R, nc, args.cpu_budget, args.cpu_threads)
     print(json.dumps(out))

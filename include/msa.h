@@ -114,6 +114,18 @@ int msa_run(msa_ctx *ctx, uint64_t t0, int64_t niter, int64_t refresh_every);
  * launches (HIP events on the context stream) in *ms. */
 int msa_run_timed(msa_ctx *ctx, uint64_t t0, int64_t niter, int64_t refresh_every, float *ms);
 
+/* Canonical Metropolis thermalization: niter iterations of Monte_Carlo with
+ * flag = 0 (main.cu:202-252, the thermal loop main.cu:124-128).  Iteration t
+ * generates the random words of parity 0 (d_gen_rndbits_cuda main.cu:647-702,
+ * counter ctr[(t*2 + 0)*R + r]) and updates colour 0 with the Metropolis
+ * branch (main.cu:945-958), then the same for parity 1 / colour 1.  `lut` is
+ * the heat-bath table of the thermal beta (dev_lut_heat_bath[0]); key[R] as
+ * for msa_set_refresh; demons are not touched.  The random bits of a site are
+ * those of its word in the reference's MSC layout (packing_u ini.c:317-337),
+ * shared by all 64 samples, so L must be a multiple of 8.  Blocks until done. */
+int msa_run_thermal(msa_ctx *ctx, int64_t niter, const msa_lut *lut, const msa_u32x4 *ctr,
+                    const uint32_t *key);
+
 /* Integer observables per sample lane (each array [..][64], NULL to skip):
  *   unsat[R][64]   unsatisfied bonds; energy h_Ener = 3V - 2*unsat
  *                  (d_computeEne main.cu:745-815, Measure_Energy_Spins main.cu:315-336)

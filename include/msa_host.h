@@ -102,6 +102,19 @@ int msa_sched_tables(const msa_sched *s, int64_t nref, int R, const uint64_t *se
                      uint32_t jobid, msa_u32x4 *ctr, uint32_t *keys, msa_lut *luts,
                      int32_t *lut_idx);
 
+/* The same tables continuing from given generator states pr[R], xo[R] (for
+ * instance those a thermal phase left, msa_thermal_counters); the caller
+ * holds the keys. */
+int msa_sched_tables_from(const msa_sched *s, int64_t nref, int R, msa_prng *pr, msa_xo256 *xo,
+                          uint32_t jobid, msa_u32x4 *ctr, msa_lut *luts, int32_t *lut_idx);
+
+/* Counters of the canonical thermalization (Monte_Carlo with flag = 0,
+ * main.cu:202-229): for t = 0 .. nterm-1 and parity p = 0, 1, _fill_bits at
+ * iteration 2t + p with parity p for every replica, ctr[(t*2 + p)*R + r].
+ * Advances each replica's generators 2*nterm times. */
+void msa_thermal_counters(msa_prng *pr, msa_xo256 *xo, int R, int64_t nterm, uint32_t jobid,
+                          msa_u32x4 *ctr);
+
 /* Shared-table form for batches and serving: temperatures are rounded to the
  * grid T = tgrid * n, n = nlo .. nlo+nlut-1, and every lattice indexes one
  * table of LUT pairs built once (msa_lut_grid).  The refresh at iteration t

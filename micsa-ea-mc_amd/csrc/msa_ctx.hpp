@@ -24,6 +24,7 @@ struct msa_ctx {
   int32_t nref = 0;
   std::vector<int32_t> lut_idx;           // host copy, indexes luts
   int32_t *meas = nullptr;                // [64]
+  uint64_t *th_rnd = nullptr;             // thermal random words [3][V/128 * R]
   uint32_t hnr = 1, hgrid = 1;            // two-pass kernel: rows per tile, persistent blocks
   int fused = 0;                          // sweeps without refresh: 0 two-pass, 1/2 one-pass kernel
   uint32_t fty = 0, ftz = 0, fny = 0;     // its tile: rows, planes, thread rows

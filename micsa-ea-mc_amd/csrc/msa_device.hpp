@@ -262,6 +262,28 @@ __device__ __forceinline__ uint64_t site_update(uint64_t s, uint64_t n0, uint64_
   return s ^ flip;
 }
 
+// The Metropolis branch of the canonical thermalization (main.cu:945-958):
+// (b0, b1, b2) are three Bernoulli(p) words, R = their leading ones
+// (P(R >= k) = p^k), and a spin flips iff n_u + R >= 3, i.e. with
+// probability min(1, exp(-beta dE)).  The flip algebra is the reference's,
+// on the local field of site_update.
+__device__ __forceinline__ uint64_t metropolis_update(uint64_t s, uint64_t n0, uint64_t n1,
+                                                      uint64_t n2, uint64_t n3, uint64_t n4,
+                                                      uint64_t n5, uint64_t J0, uint64_t J1,
+                                                      uint64_t J2, uint64_t J3, uint64_t J4,
+                                                      uint64_t J5, uint64_t b0, uint64_t b1,
+                                                      uint64_t b2) {
+  const uint64_t f0 = xor3(s, J0, n0), f1 = xor3(s, J1, n1), f2 = xor3(s, J2, n2);
+  const uint64_t j1 = xor3(f0, f1, f2), j2 = maj3(f0, f1, f2);
+  const uint64_t f3 = xor3(s, J3, n3), f4 = xor3(s, J4, n4), f5 = xor3(s, J5, n5);
+  const uint64_t j3 = xor3(f3, f4, f5), j4 = maj3(f3, f4, f5);
+  b1 &= b0;
+  b2 &= b1;
+  const uint64_t id2 = b1, id1 = (b0 ^ b1) | b2, j24 = j2 | j4;
+  const uint64_t flip = (j1 & j3 & id1) | ((j24 | id2) & (j1 | j3 | id1)) | ((j2 & j4) | (j24 & id2));
+  return s ^ flip;
+}
+
 // --------------------------------------------------- per-lane counting --
 // Bit-sliced (vertical) counter: word j holds bit j of 64 independent lane
 // counts.  add() is a ripple-carry add of a 1-bit-per-lane word at weight 2^pos.

@@ -42,7 +42,11 @@ struct SweepArgs {
   const uint4 *ctr;         // refresh: [R] counters of this refresh
   const uint32_t *key;      // refresh: [R]
   const msa::FineLut *lut;  // refresh: [2] plane-0 and plane-1 tables of this refresh
+  const uint64_t *rnd;      // thermal: [3][rT] random words of k_gen_rndbits
+  uint32_t rT;              // thermal: V/128 * R
 };
+
+enum HalfKind { kMicro = 0, kRefresh = 1, kThermal = 2 };
 
 // floor(n / d) for n * d < 2^32 with m = floor(2^32 / d) + 1 (div_recip_of):
 // n * m / 2^32 exceeds n / d by less than n / 2^32 < 1 / d, so the floor is exact.
@@ -51,16 +55,22 @@ __device__ __forceinline__ uint32_t div_recip(uint32_t n, uint32_t m) { return _
 
 // One colour of a sweep with demon layer `layer` (d_oneMCstepBN_multisample,
 // main.cu:847-1080, ISI branch), with the demon walk of that layer fused into
-// the demon store and, with REFRESH, the heat-bath refresh of the layer fused
+// the demon store and, for kRefresh, the heat-bath refresh of the layer fused
 // into the demon load.  Demons go dm_in -> dm_out and spins sp_own -> sp_out,
 // so neither the walk's permutation nor the update races with a load.
+// kThermal is the same kernel's Metropolis branch (main.cu:945-958) of the
+// canonical thermalization: no demons; site (x, y, z) takes bit
+// bx + 4 by + 16 bz of the random words of its MSC word
+// (a + ML be + ML^2 g) / 2 and replica (packing_u ini.c:317-337, ML = L/4),
+// the same bit for every sample, as rand_h is indexed in main.cu:877-879.
 // Thread = one site of the colour, all R replicas (J loaded once).  Blocks
 // are persistent: each stages the two search tables of the refresh in LDS
 // once (2 x 18 KiB) and then walks tiles of nr rows.  Used for every sweep
-// with a refresh (VALU-bound: Philox and the table search) and for the
-// two-pass mode.
-template <bool REFRESH>
+// with a refresh (VALU-bound: Philox and the table search), for the
+// two-pass mode and for thermal sweeps.
+template <int KIND>
 __global__ void __launch_bounds__(1024) k_halfsweep(SweepArgs a) {
+  constexpr bool REFRESH = KIND == kRefresh;
   extern __shared__ uint4 s_tab[];
   const Geo g = a.g;
   const msa::FineLut *tabs = reinterpret_cast<const msa::FineLut *>(s_tab);
@@ -96,6 +106,28 @@ __global__ void __launch_bounds__(1024) k_halfsweep(SweepArgs a) {
     const uint64_t hw = msa::walk_dest(g, col, dx, dy, dz, i, y, z);
     const uint32_t tid = (uint32_t)((uint64_t)a.layer * g.V + x + (uint64_t)L * row);
     uint64_t J0, J1, J2, J3, J4, J5;
+    if (KIND == kThermal) {
+      const uint32_t ML = L >> 2;
+      const uint32_t bx = (x >= ML) + (x >= 2 * ML) + (x >= 3 * ML);
+      const uint32_t by = (y >= ML) + (y >= 2 * ML) + (y >= 3 * ML);
+      const uint32_t bz = (z >= ML) + (z >= 2 * ML) + (z >= 3 * ML);
+      const uint32_t ms = (x - bx * ML) + ML * ((y - by * ML) + ML * (z - bz * ML));
+      const uint32_t bit = bx + 4 * by + 16 * bz;
+      J0 = Jc[h]; J1 = Jc[H + h]; J2 = Jc[2 * H + h];
+      J3 = Jo[hxm]; J4 = Jo[H + hym]; J5 = Jo[2 * H + hzm];
+      for (uint32_t r = 0; r < g.R; ++r) {
+        const uint32_t w = (ms >> 1) * g.R + r;
+        const uint64_t b0 = 0ull - ((a.rnd[w] >> bit) & 1ull);
+        const uint64_t b1 = 0ull - ((a.rnd[a.rT + w] >> bit) & 1ull);
+        const uint64_t b2 = 0ull - ((a.rnd[2 * (uint64_t)a.rT + w] >> bit) & 1ull);
+        const uint64_t *So = a.sp_nb + (uint64_t)(2 * r + (col ^ 1)) * g.H;
+        const uint64_t sown = a.sp_own[(uint64_t)(2 * r + col) * g.H + h];
+        a.sp_out[(uint64_t)(2 * r + col) * g.H + h] =
+            msa::metropolis_update(sown, So[hxp], So[hyp], So[hzp], So[hxm], So[hym], So[hzm], J0,
+                                   J1, J2, J3, J4, J5, b0, b1, b2);
+      }
+      continue;
+    }
     for (uint32_t r = 0; r < g.R; ++r) {
       const uint64_t *So = a.sp_nb + (uint64_t)(2 * r + (col ^ 1)) * g.H;
       const uint64_t dbase = ((uint64_t)(r * g.nc + a.layer) * 2 + col) * g.H;
@@ -146,6 +178,31 @@ __global__ void __launch_bounds__(256) k_fine_luts(const msa_lut *luts, uint32_t
   }
   out[tab].pre[b] = (uint8_t)lo;
   if (b < 512) out[tab].umb[b] = u[b];
+}
+
+// d_gen_rndbits_cuda (main.cu:647-702) with NBETAS = 1: thread tid < T =
+// (V/128) R, replica tid % R (main.cu:664-665), counter ctr[ir] with
+// .z += tid, key {tid, key[ir]}, six words from the heat-bath table:
+// out[k T + tid] = w_{2k} << 32 | w_{2k+1} (main.cu:683-700).
+__global__ void __launch_bounds__(256) k_gen_rndbits(const uint4 *ctr, const uint32_t *key,
+                                                     const msa::FineLut *lut, uint32_t R,
+                                                     uint32_t T, uint64_t *out) {
+  __shared__ uint4 s_tab[sizeof(msa::FineLut) / 16];
+  const uint4 *src = reinterpret_cast<const uint4 *>(lut);
+  for (uint32_t k = threadIdx.x; k < sizeof(msa::FineLut) / 16; k += blockDim.x) s_tab[k] = src[k];
+  __syncthreads();
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= T) return;
+  const msa::LutView t = msa::lut_view(reinterpret_cast<const msa::FineLut *>(s_tab));
+  const uint32_t ir = tid % R;
+  const uint4 c4 = ctr[ir];
+  uint32_t c[4] = {c4.x, c4.y, c4.z + tid, c4.w};
+  uint64_t w[6];
+#pragma unroll
+  for (int j = 0; j < 6; ++j) w[j] = msa::rnd_word(t, c, tid, key[ir]);
+  out[tid] = (w[0] << 32) | w[1];
+  out[T + tid] = (w[2] << 32) | w[3];
+  out[2 * (uint64_t)T + tid] = (w[4] << 32) | w[5];
 }
 
 int msa_build_fine_luts(hipStream_t st, int32_t nlut, const msa_lut *host, msa::FineLut **out) {
@@ -681,7 +738,7 @@ extern "C" int msa_create(int device, const msa_geom *geom, msa_ctx **out) {
       hipStreamSynchronize(c->stream) != hipSuccess)
     return cleanup(msa_fail(MSA_ERR_HIP, "initial clear failed"));
   // the refresh kernel stages two search tables (37 KiB) in LDS
-  if (hipFuncSetAttribute(reinterpret_cast<const void *>(&k_halfsweep<true>),
+  if (hipFuncSetAttribute(reinterpret_cast<const void *>(&k_halfsweep<kRefresh>),
                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)kRefreshLds) != hipSuccess)
     return cleanup(msa_fail(MSA_ERR_HIP, "refresh kernel LDS attribute"));
   pick_half(c);
@@ -703,6 +760,7 @@ extern "C" int msa_destroy(msa_ctx *c) {
   (void)hipFree(c->ctr);
   (void)hipFree(c->key);
   (void)hipFree(c->meas);
+  (void)hipFree(c->th_rnd);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -968,9 +1026,9 @@ static int launch_run(msa_ctx *c, uint64_t t0, int64_t niter, int64_t every) {
           a.sp_nb = col ? B : A;
           a.sp_out = B;
           if (k >= 0)
-            hipLaunchKernelGGL(k_halfsweep<true>, grid, block, kRefreshLds, c->stream, a);
+            hipLaunchKernelGGL(k_halfsweep<kRefresh>, grid, block, kRefreshLds, c->stream, a);
           else
-            hipLaunchKernelGGL(k_halfsweep<false>, grid, block, 0, c->stream, a);
+            hipLaunchKernelGGL(k_halfsweep<kMicro>, grid, block, 0, c->stream, a);
         }
       }
       c->scur ^= 1;
@@ -998,6 +1056,66 @@ extern "C" int msa_run_timed(msa_ctx *c, uint64_t t0, int64_t niter, int64_t ref
   HIPCHK(hipEventSynchronize(c->ev1));
   HIPCHK(hipEventElapsedTime(ms, c->ev0, c->ev1));
   return MSA_OK;
+}
+
+extern "C" int msa_run_thermal(msa_ctx *c, int64_t niter, const msa_lut *lut, const msa_u32x4 *ctr,
+                               const uint32_t *key) {
+  if (!c || !lut || (niter > 0 && (!ctr || !key))) return msa_fail(MSA_ERR_ARG, "NULL argument");
+  if (niter < 0) return msa_fail(MSA_ERR_ARG, "niter=%lld", (long long)niter);
+  if (c->g.L % 8) return msa_fail(MSA_ERR_ARG, "thermal sweeps use the MSC word map: L=%u must be a multiple of 8", c->g.L);
+  if (!lut_valid(*lut)) return msa_fail(MSA_ERR_ARG, "heat-bath LUT is malformed");
+  if (niter == 0) return MSA_OK;
+  HIPCHK(hipSetDevice(c->device));
+  const Geo &g = c->g;
+  const uint32_t T = (uint32_t)(g.V / 128 * g.R);
+  if (!c->th_rnd && hipMalloc(&c->th_rnd, 3ull * T * sizeof(uint64_t)) != hipSuccess)
+    return msa_fail(MSA_ERR_NOMEM, "thermal random words");
+  // the table twice: msa_build_fine_luts builds pairs
+  const msa_lut pair[2] = {*lut, *lut};
+  msa::FineLut *tab = nullptr;
+  int rc = msa_build_fine_luts(c->stream, 1, pair, &tab);
+  if (rc) return rc;
+  uint4 *dctr = nullptr;
+  uint32_t *dkey = nullptr;
+  const size_t nctr = (size_t)niter * 2 * g.R;
+  if (hipMalloc(&dctr, sizeof(uint4) * nctr) != hipSuccess || hipMalloc(&dkey, sizeof(uint32_t) * g.R) != hipSuccess) {
+    rc = msa_fail(MSA_ERR_NOMEM, "thermal counters");
+  } else if (hipMemcpyAsync(dctr, ctr, sizeof(uint4) * nctr, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+             hipMemcpyAsync(dkey, key, sizeof(uint32_t) * g.R, hipMemcpyHostToDevice, c->stream) != hipSuccess) {
+    rc = msa_fail(MSA_ERR_HIP, "thermal counter upload");
+  } else {
+    SweepArgs a{};
+    a.jp = c->jp;
+    a.g = g;
+    a.nr = c->hnr;
+    a.mLh = div_recip_of(g.Lh);
+    a.mL = div_recip_of(g.L);
+    a.rnd = c->th_rnd;
+    a.rT = T;
+    const dim3 grid(c->hgrid), block(c->hnr * g.Lh);
+    for (int64_t t = 0; t < niter && !rc; ++t) {
+      const uint64_t *A = c->sp[c->scur];
+      uint64_t *B = c->sp[c->scur ^ 1];
+      for (int col = 0; col < 2; ++col) {
+        // Monte_Carlo flag = 0 (main.cu:202-252): bits of parity col, then colour col
+        hipLaunchKernelGGL(k_gen_rndbits, dim3((T + 255) / 256), dim3(256), 0, c->stream,
+                           dctr + ((size_t)t * 2 + col) * g.R, dkey, tab, g.R, T, c->th_rnd);
+        a.colour = col;
+        a.sp_own = A;
+        a.sp_nb = col ? B : A;
+        a.sp_out = B;
+        hipLaunchKernelGGL(k_halfsweep<kThermal>, grid, block, 0, c->stream, a);
+      }
+      if (hipGetLastError() != hipSuccess) rc = msa_fail(MSA_ERR_HIP, "thermal launch");
+      c->scur ^= 1;
+    }
+    if (hipStreamSynchronize(c->stream) != hipSuccess && !rc) rc = msa_fail(MSA_ERR_HIP, "thermal sweeps");
+  }
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(dctr);
+  (void)hipFree(dkey);
+  (void)hipFree(tab);
+  return rc;
 }
 
 static int measure_one(msa_ctx *c, int what, int ra, int rb, int32_t *host_out) {

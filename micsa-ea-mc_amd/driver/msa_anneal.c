@@ -22,9 +22,12 @@
  *                0, 1, 2, 4, 8, ... as the reference does (main.cu:140-142)
  *
  * Differences from the reference, by design: demons are refreshed on the GPU
- * from Philox4x32-10 + LUT bits (no host threads); Metropolis thermalization
- * (nterm > 0) and restart from checkpoints (flag = 0) are not part of this
- * path and are rejected with an error.
+ * from Philox4x32-10 + LUT bits (no host threads); restart from checkpoints
+ * (flag = 0) is not part of this path and is rejected with an error.  The
+ * canonical Metropolis thermalization (nterm > 0, main.cu:124-128) runs on
+ * the GPU with the LUT file's heat-bath table, its counters drawn from the
+ * per-replica generators before those of the demon refreshes, as the
+ * reference's _fill_bits calls consume the same generators.
  */
 #include <errno.h>
 #include <math.h>
@@ -110,7 +113,8 @@ int main(int argc, char **argv)
   if (msa_read_samples(f_samples, nbits, samples, e, sizeof e)) die("%s", e);
   for (int i = 0; i < nbits; i++)
     if (samples[i] < 0 || samples[i] > 63) die("sample lane %d outside 0..63", samples[i]);
-  if (in.nterm > 0) die("nterm=%d: Metropolis thermalization is not part of this path", in.nterm);
+  if (in.nterm < 0) die("nterm=%d must not be negative", in.nterm);
+  if (in.nterm > 0 && ((in.l & 0xffff) % 8)) die("nterm=%d: thermalization needs L to be a multiple of 8", in.nterm);
   if (in.flag == 0 || in.flag < -2 || in.flag > 2) die("flag=%d: only new runs (flag 1 or 2) are supported", in.flag);
   if (in.nc < 1 || in.nc > 6) die("nc=%d outside 1..6", in.nc);
   const int L = in.l & 0xffff, R = in.nr;
@@ -165,7 +169,20 @@ int main(int argc, char **argv)
   int32_t *lut_idx = (int32_t *)malloc(sizeof(int32_t) * (size_t)nref);
   uint32_t keys[MSA_MAXR];
   if (!ctr || !luts || !lut_idx) die("Problems allocating refresh tables (%lld)", (long long)nref);
-  msa_sched_tables(&sch, nref, R, seed_MC, (uint32_t)sample, ctr, keys, luts, lut_idx);
+  msa_prng pr[MSA_MAXR];
+  msa_xo256 xo[MSA_MAXR];
+  for (int r = 0; r < R; r++) msa_cpu_generators(&pr[r], &xo[r], &keys[r], seed_MC[r]);
+  if (in.nterm > 0) {
+    /* THERMALIZATION, main.cu:124-128: Monte_Carlo(flag = 0) nterm times */
+    printf("Thermalization: %d Metropolis iterations at beta = %.6f\n", in.nterm, beta);
+    fflush(stdout);
+    msa_u32x4 *tctr = (msa_u32x4 *)malloc(sizeof(msa_u32x4) * 2 * (size_t)in.nterm * R);
+    if (!tctr) die("Problems allocating thermal counters");
+    msa_thermal_counters(pr, xo, R, in.nterm, (uint32_t)sample, tctr);
+    CHK(msa_run_thermal(ctx, in.nterm, &lut_file, tctr, keys));
+    free(tctr);
+  }
+  msa_sched_tables_from(&sch, nref, R, pr, xo, (uint32_t)sample, ctr, luts, lut_idx);
   CHK(msa_set_luts(ctx, (int32_t)nref, luts));
   CHK(msa_set_refresh(ctx, (int32_t)nref, ctr, keys, lut_idx));
   free(luts);

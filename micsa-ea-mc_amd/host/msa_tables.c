@@ -100,6 +100,20 @@ double msa_sched_beta(const msa_sched *s, int64_t k)
   return 1.0 / T;
 }
 
+int msa_sched_tables_from(const msa_sched *s, int64_t nref, int R, msa_prng *pr, msa_xo256 *xo,
+                          uint32_t jobid, msa_u32x4 *ctr, msa_lut *luts, int32_t *lut_idx)
+{
+  if (R < 1 || R > MSA_MAXR || nref < 0) return -1;
+  for (int64_t k = 0; k < nref; k++) {
+    const uint64_t t = (uint64_t)msa_sched_time(s, k);
+    for (int r = 0; r < R; r++) msa_fill_counter(&pr[r], &xo[r], t, 0, jobid, &ctr[k * R + r]);
+    lut_idx[k] = (int32_t)k;
+  }
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t k = 0; k < nref; k++) msa_lut_demon(msa_sched_beta(s, k), &luts[2 * k], &luts[2 * k + 1]);
+  return 0;
+}
+
 int msa_sched_tables(const msa_sched *s, int64_t nref, int R, const uint64_t *seed_MC,
                      uint32_t jobid, msa_u32x4 *ctr, uint32_t *keys, msa_lut *luts,
                      int32_t *lut_idx)
@@ -109,14 +123,16 @@ int msa_sched_tables(const msa_sched *s, int64_t nref, int R, const uint64_t *se
   msa_xo256 xo[MSA_MAXR];
 #pragma omp parallel for schedule(static)
   for (int r = 0; r < R; r++) msa_cpu_generators(&pr[r], &xo[r], &keys[r], seed_MC[r]);
-  for (int64_t k = 0; k < nref; k++) {
-    const uint64_t t = (uint64_t)msa_sched_time(s, k);
-    for (int r = 0; r < R; r++) msa_fill_counter(&pr[r], &xo[r], t, 0, jobid, &ctr[k * R + r]);
-    lut_idx[k] = (int32_t)k;
-  }
-#pragma omp parallel for schedule(dynamic, 16)
-  for (int64_t k = 0; k < nref; k++) msa_lut_demon(msa_sched_beta(s, k), &luts[2 * k], &luts[2 * k + 1]);
-  return 0;
+  return msa_sched_tables_from(s, nref, R, pr, xo, jobid, ctr, luts, lut_idx);
+}
+
+void msa_thermal_counters(msa_prng *pr, msa_xo256 *xo, int R, int64_t nterm, uint32_t jobid,
+                          msa_u32x4 *ctr)
+{
+  for (int64_t t = 0; t < nterm; t++)
+    for (uint32_t p = 0; p < 2; p++)
+      for (int r = 0; r < R; r++)
+        msa_fill_counter(&pr[r], &xo[r], (uint64_t)(2 * t + p), p, jobid, &ctr[(t * 2 + p) * R + r]);
 }
 
 /* ------------------------------------------------- shared grid tables -- */

@@ -70,6 +70,7 @@ def hip():
         L.msa_sync.argtypes = [vp]
         L.msa_set_tiling.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32]
         L.msa_get_tiling.argtypes = [vp, i32p, i32p, i32p]
+        L.msa_run_thermal.argtypes = [vp, C.c_int64, C.c_void_p, u32p, u32p]
         bp = C.c_void_p
         L.msa_batch_create.argtypes = [C.c_int, C.c_int64, C.POINTER(bp)]
         L.msa_batch_destroy.argtypes = [bp]
@@ -169,6 +170,16 @@ class Lattice:
     def run(self, t0, niter, refresh_every):
         _check(self.lib.msa_run(self.ctx, t0, niter, refresh_every))
 
+    def run_thermal(self, lut, ctr, key):
+        """Canonical Metropolis thermalization (msa_run_thermal): `lut` is the
+        4096 bytes of the heat-bath table, ctr (nterm, 2, R, 4), key (R,)."""
+        raw = np.ascontiguousarray(np.frombuffer(bytes(lut), np.uint8))
+        assert raw.size == LUT_BYTES
+        ctr = np.ascontiguousarray(ctr, np.uint32).reshape(-1, 2, self.R, 4)
+        key = np.ascontiguousarray(key, np.uint32)
+        _check(self.lib.msa_run_thermal(self.ctx, ctr.shape[0], raw.ctypes.data_as(C.c_void_p),
+                                        _p(ctr, u32p), _p(key, u32p)))
+
     def run_timed(self, t0, niter, refresh_every):
         ms = C.c_float()
         _check(self.lib.msa_run_timed(self.ctx, t0, niter, refresh_every, C.byref(ms)))
@@ -266,6 +277,10 @@ def host():
         L.msa_sched_tables.argtypes = [C.POINTER(Sched), C.c_int64, C.c_int, u64p, C.c_uint32,
                                        u32p, u32p, C.c_void_p, i32p]
         L.msa_lut_grid.argtypes = [C.c_double, C.c_int32, C.c_int32, C.c_void_p]
+        L.msa_thermal_counters.argtypes = [C.POINTER(Prng), C.POINTER(Xo256), C.c_int, C.c_int64,
+                                           C.c_uint32, u32p]
+        L.msa_sched_tables_from.argtypes = [C.POINTER(Sched), C.c_int64, C.c_int, C.POINTER(Prng),
+                                            C.POINTER(Xo256), C.c_uint32, u32p, C.c_void_p, i32p]
         L.msa_sched_tables_grid.argtypes = [C.POINTER(Sched), C.c_int64, C.c_int, u64p, C.c_uint32,
                                             C.c_double, C.c_int32, C.c_int32, u32p, u32p, i32p]
         _host = L
@@ -320,6 +335,41 @@ def build_state(L, R, seed_J, seed_u):
 
 def schedule(T0, T1, nsweeps, refresh_every, beta_fixed=0.0):
     return Sched(T0, T1, nsweeps, refresh_every, beta_fixed)
+
+
+def generators(seed_mc):
+    """Per-replica (Prng, Xo256) states and keys (Inicia_generadores_CPU)."""
+    R = len(seed_mc)
+    pr, xo = (Prng * R)(), (Xo256 * R)()
+    keys = np.zeros(R, np.uint32)
+    for r, sd in enumerate(seed_mc):
+        host().msa_cpu_generators(C.byref(pr[r]), C.byref(xo[r]), _p(keys[r:r + 1], u32p), sd)
+    return pr, xo, keys
+
+
+def thermal_counters(gens, nterm, jobid=0):
+    """Counters (nterm, 2, R, 4) of the thermal phase; advances `gens`."""
+    pr, xo, keys = gens
+    R = len(keys)
+    ctr = np.zeros((max(nterm, 1), 2, R, 4), np.uint32)
+    host().msa_thermal_counters(pr, xo, R, nterm, jobid, _p(ctr, u32p))
+    return ctr[:nterm]
+
+
+def schedule_tables_from(sched, niter, gens, jobid=0):
+    """schedule_tables continuing from generator states (after a thermal phase)."""
+    pr, xo, keys = gens
+    R = len(keys)
+    H = host()
+    nref = H.msa_sched_count(C.byref(sched), niter)
+    ctr = np.zeros((nref, R, 4), np.uint32)
+    luts = np.zeros(2 * nref * LUT_BYTES, np.uint8)
+    idx = np.zeros(nref, np.int32)
+    rc = H.msa_sched_tables_from(C.byref(sched), nref, R, pr, xo, jobid, _p(ctr, u32p),
+                                 luts.ctypes.data_as(C.c_void_p), _p(idx, i32p))
+    if rc:
+        raise MsaError("schedule tables failed")
+    return ctr, keys.copy(), luts, idx
 
 
 def schedule_tables(sched, niter, R, seed_mc, jobid=0):

@@ -66,6 +66,36 @@ class Tables:
         return msa.Batch.make_job(req.L, req.R, req.seed, req.niter, 1, ctr, keys, idx)
 
 
+BLOCKS = (256, 512, 1024)
+
+
+def tune_block(tables, reqs, iters=8, candidates=BLOCKS, jobs=None):
+    """Autotuner of the batched launch shape: threads per block of the batch
+    kernel, timed on a scratch batch holding `reqs` (the workload or a sample
+    of it; `jobs` may give their prepared msa_job) for `iters` iterations after
+    two of warm-up.  Returns (best block, {block: seconds})."""
+    times = {}
+    jobs = jobs or [tables.job(r, i) for i, r in enumerate(reqs)]
+    for block in candidates:
+        b = msa.Batch(tables.luts)
+        b.set_shape(block, iters)
+        for j in jobs:
+            b.add(j)
+        b.run(2)
+        b.sync()
+        t0 = time.perf_counter()
+        b.run(iters)
+        b.sync()
+        times[block] = time.perf_counter() - t0
+        b.close()
+    return min(times, key=times.get), times
+
+
+def size_sample(sizes=(16, 32, 48, 64, 96, 128), niter=64, T0=2.0, T1=0.7):
+    """One request per lattice size: the sample the served loop tunes on."""
+    return [Request(i, L, niter, T0, T1, 7 + i, 0.0) for i, L in enumerate(sizes)]
+
+
 def run_alone(req, tables, jobid):
     """The request by itself (a batch holding only it): the reference result."""
     b = msa.Batch(tables.luts)
@@ -77,9 +107,13 @@ def run_alone(req, tables, jobid):
     return out
 
 
-def serve(requests, max_bytes, epoch=64, block=256, tables=None, keep_state=False, clock=None):
-    """Run the stream; returns (per-request dicts, summary dict)."""
+def serve(requests, max_bytes, epoch=64, block=None, tables=None, keep_state=False, clock=None):
+    """Run the stream; returns (per-request dicts, summary dict).  block=None
+    picks the batch kernel's block size with tune_block on one request per
+    lattice size of the stream."""
     tables = tables or Tables()
+    if block is None:
+        block, _ = tune_block(tables, size_sample(sorted({r.L for r in requests})))
     batch = msa.Batch(tables.luts, max_bytes=max_bytes)
     batch.set_shape(block, epoch)
     reqs = sorted(requests, key=lambda r: r.arrival)
@@ -134,5 +168,5 @@ def serve(requests, max_bytes, epoch=64, block=256, tables=None, keep_state=Fals
                "latency_p50_s": float(np.percentile(lat, 50)),
                "latency_p99_s": float(np.percentile(lat, 99)),
                "max_concurrent": max_active, "max_bytes_in_use": max_bytes_used,
-               "epoch": epoch}
+               "epoch": epoch, "block": block}
     return done, summary

@@ -246,6 +246,58 @@ static void msc_mc(orc_msc *m, int dir, int ic)
     }
 }
 
+/* d_oneMCstepBN_multisample with rand_h != 0 (main.cu:847-958): the
+ * Metropolis branch, one colour; rand_h is offset by ibeta*(n*nrep) +
+ * tidModNNRep = word*R + replica (main.cu:877-879) and read at strides of
+ * the generator's thread count (here n*R). */
+static void msc_mc_metropolis(orc_msc *m, int dir, const uint64_t *rnd)
+{
+  const int R = m->R, HALF = m->HALF;
+  const size_t T = (size_t)HALF * R;
+  uint64_t *newS = m->spin[dir], *oldS = m->spin[dir ? 0 : 1];
+  const uint64_t *J = dir ? m->JN : m->JB;
+  const int *nbr = dir ? m->vicN : m->vicB;
+  const uint8_t *rot = dir ? m->rotN : m->rotB;
+  for (int ws = 0; ws < HALF; ws++)
+    for (int wr = 0; wr < R; wr++) {
+      uint64_t s = newS[ws * R + wr];
+      const uint64_t *j = J + 6 * ws;
+      uint64_t c0 = oldS[nbr[6 * ws + 0] * R + wr];
+      uint64_t c1 = oldS[nbr[6 * ws + 1] * R + wr];
+      uint64_t c2 = oldS[nbr[6 * ws + 2] * R + wr];
+      uint64_t c3 = oldS[nbr[6 * ws + 3] * R + wr] ^ j[3];
+      uint64_t c4 = oldS[nbr[6 * ws + 4] * R + wr] ^ j[4];
+      uint64_t c5 = oldS[nbr[6 * ws + 5] * R + wr] ^ j[5];
+      uint8_t r = rot[ws];
+      if (r & 0x01) c0 = rotE(c0);
+      if (r & 0x02) c1 = rotN(c1);
+      if (r & 0x04) c2 = rotU(c2);
+      if (r & 0x08) c3 = rotO(c3);
+      if (r & 0x10) c4 = rotS(c4);
+      if (r & 0x20) c5 = rotD(c5);
+      uint64_t f0 = s ^ j[0] ^ c0, f1 = s ^ j[1] ^ c1, f2 = s ^ j[2] ^ c2;
+      uint64_t k1 = f0 ^ f1, k2 = f0 & f1, j1 = k1 ^ f2, k3 = k1 & f2, j2 = k2 ^ k3;
+      f0 = c3 ^ s; f1 = c4 ^ s; f2 = c5 ^ s;
+      k1 = f0 ^ f1; k2 = f0 & f1; k3 = k1 & f2;
+      uint64_t j3 = k1 ^ f2, j4 = k2 ^ k3;
+      const size_t w = (size_t)ws * R + wr;
+      newS[w] = s ^ orc_metropolis_flip(j1, j2, j3, j4, rnd[w], rnd[T + w], rnd[2 * T + w]);
+    }
+}
+
+/* Monte_Carlo flag=0, main.cu:202-252: d_gen_rndbits_cuda (orc_gen_rndbits)
+ * with the parity-0 counter, the colour-0 update, then the same for parity 1. */
+void orc_msc_thermal(orc_msc *m, const orc_lut *lut, const uint32_t *ctr0, const uint32_t *ctr1,
+                     const uint32_t *key)
+{
+  uint64_t *rnd = (uint64_t *)malloc(3 * (size_t)m->HALF * m->R * sizeof(uint64_t));
+  orc_gen_rndbits(m->L, m->R, lut, ctr0, key, rnd);
+  msc_mc_metropolis(m, 0, rnd);
+  orc_gen_rndbits(m->L, m->R, lut, ctr1, key, rnd);
+  msc_mc_metropolis(m, 1, rnd);
+  free(rnd);
+}
+
 /* d_Walk, main.cu:1084-1193, one colour pass over all layers */
 static void msc_walk(orc_msc *m, int dir)
 {

@@ -29,6 +29,13 @@
  *                       reference has no device refresh); its pieces
  *                       (Philox, counter, LUT, bisection) are pinned as above
  *                       and its law is checked statistically.
+ *   orc_metropolis_flip exhaustive closed form: flip iff n_u + R >= 3 with R
+ *                       the leading ones of (b0, b1, b2).
+ *   orc_gen_rndbits     built from pinned pieces (counter, Philox, LUT,
+ *                       bisection) with the thread indexing of
+ *                       main.cu:647-702; its law is checked statistically.
+ *   orc_thermal_*       the MSC-layout transcription orc_msc_thermal
+ *                       (main.cu:202-252, 847-958) on the same state.
  */
 #ifndef MSA_ORACLE_H
 #define MSA_ORACLE_H
@@ -127,6 +134,15 @@ void orc_demon_energy(const orc_lattice *lat, int32_t *ed /*[R][64]*/);
 void orc_overlap(const orc_lattice *lat, int ra, int rb, int32_t *q /*[64]*/,
                  int32_t *ql /*[64]*/);
 
+/* canonical Metropolis thermalization, Monte_Carlo flag = 0 (thermal.c) */
+void     orc_gen_rndbits(int L, int R, const orc_lut *lut, const uint32_t *ctr /*[R][4]*/,
+                         const uint32_t *key /*[R]*/, uint64_t *out /*[3][V/128*R]*/);
+uint64_t orc_metropolis_flip(uint64_t j1, uint64_t j2, uint64_t j3, uint64_t j4, uint64_t b0,
+                             uint64_t b1, uint64_t b2);
+void     orc_thermal_halfsweep(orc_lattice *lat, int colour, const uint64_t *rnd);
+void     orc_thermal_iteration(orc_lattice *lat, const orc_lut *lut, const uint32_t *ctr0,
+                               const uint32_t *ctr1, const uint32_t *key);
+
 /* state construction (ini.c:275-299, ini.c:457-471, io.c:794-796) */
 void orc_init_spins_ref(orc_lattice *lat, uint64_t seed_u);
 void orc_init_J_ref(orc_lattice *lat, uint64_t seed_J);
@@ -143,6 +159,8 @@ void     orc_msc_pack(orc_msc *m, const uint8_t *spin, const uint8_t *jx, const 
                       const uint8_t *jz, const uint8_t *d0, const uint8_t *d1);
 void     orc_msc_unpack(const orc_msc *m, uint8_t *spin, uint8_t *d0, uint8_t *d1);
 void     orc_msc_monte_carlo(orc_msc *m);       /* main.cu:254-306 (micro branch) */
+void     orc_msc_thermal(orc_msc *m, const orc_lut *lut, const uint32_t *ctr0,
+                         const uint32_t *ctr1, const uint32_t *key);   /* main.cu:202-252 */
 void     orc_msc_energy(const orc_msc *m, int32_t *e /*[R]*/);
 int      orc_msc_scalar_energy(int L, const uint8_t *spin, const uint8_t *jx,
                                const uint8_t *jy, const uint8_t *jz);

@@ -128,6 +128,11 @@ class Oracle:
         L.orc_init_spins_ref.argtypes = [lat, C.c_uint64]
         L.orc_init_J_ref.argtypes = [lat, C.c_uint64]
         L.orc_init_philox.argtypes = [lat, C.c_uint64]
+        L.orc_gen_rndbits.argtypes = [C.c_int, C.c_int, C.POINTER(Lut), u32p, u32p, u64p]
+        L.orc_metropolis_flip.argtypes = [C.c_uint64] * 7
+        L.orc_metropolis_flip.restype = C.c_uint64
+        L.orc_thermal_iteration.argtypes = [lat, C.POINTER(Lut), u32p, u32p, u32p]
+        L.orc_msc_thermal.argtypes = [C.c_void_p, C.POINTER(Lut), u32p, u32p, u32p]
         L.orc_msc_create.argtypes = [C.c_int, C.c_int, C.c_int]
         L.orc_msc_create.restype = C.c_void_p
         L.orc_msc_destroy.argtypes = [C.c_void_p]
@@ -265,6 +270,24 @@ class Oracle:
 
     def init_philox(self, lat, seed):
         self.lib.orc_init_philox(lat.struct(), seed)
+
+    # ---- canonical Metropolis thermalization
+    def gen_rndbits(self, L, R, lut, ctr, key):
+        out = np.zeros((3, L ** 3 // 128 * R), np.uint64)
+        ctr = np.ascontiguousarray(ctr, np.uint32)
+        key = np.ascontiguousarray(key, np.uint32)
+        self.lib.orc_gen_rndbits(L, R, C.byref(lut), _p(ctr, u32p), _p(key, u32p), _p(out))
+        return out
+
+    def metropolis_flip(self, j1, j2, j3, j4, b0, b1, b2):
+        return self.lib.orc_metropolis_flip(j1, j2, j3, j4, b0, b1, b2)
+
+    def thermal_iteration(self, lat, lut, ctr0, ctr1, key):
+        ctr0 = np.ascontiguousarray(ctr0, np.uint32)
+        ctr1 = np.ascontiguousarray(ctr1, np.uint32)
+        key = np.ascontiguousarray(key, np.uint32)
+        self.lib.orc_thermal_iteration(lat.struct(), C.byref(lut), _p(ctr0, u32p), _p(ctr1, u32p),
+                                       _p(key, u32p))
 
 
 def build():

@@ -38,7 +38,8 @@ def run_driver(d, extra=(), sample=3, nbits=2):
 
 
 @pytest.mark.parametrize("change,msg", [
-    (dict(nterm=5), "Metropolis"),
+    (dict(nterm=5, L=12), "multiple of 8"),
+    (dict(nterm=-1), "Bad data input"),
     (dict(flag=0), "only new runs"),
     (dict(beta="0.5"), "betas[0]"),
     (dict(nc=7), "nc=7"),
@@ -59,13 +60,13 @@ def test_driver_usage(tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("use_sched", [False, True])
-def test_driver_end_to_end_matches_oracle(tmp_path, orc, pkg, use_sched):
-    from sched_util import oracle_schedule
+@pytest.mark.parametrize("use_sched,nterm", [(False, 0), (True, 0), (False, 3)])
+def test_driver_end_to_end_matches_oracle(tmp_path, orc, pkg, use_sched, nterm):
+    from sched_util import oracle_schedule, oracle_thermal
     d = str(tmp_path)
     L, nr, maxtime = 8, 2, 16
     times = (1, 2, 4, 8, 16)
-    out = write_inputs(d, L=L, nr=nr, maxtime=maxtime, times=times)
+    out = write_inputs(d, L=L, nr=nr, maxtime=maxtime, times=times, nterm=nterm)
     extra = []
     if use_sched:
         with open(os.path.join(d, "sched.in"), "w") as f:
@@ -78,13 +79,19 @@ def test_driver_end_to_end_matches_oracle(tmp_path, orc, pkg, use_sched):
     orc.init_ref(o, 1001, 2002)
     o.D0[...] = 0
     o.D1[...] = 0
+    gens, tctr = oracle_thermal(orc, nr, nterm, seed_mc=3003, jobid=3)
+    if nterm:
+        # thermalization at the LUT file's beta (main.cu:124-128), then the micro phase
+        lut = orc.lut_beta(0.90909090909090)
+        for t in range(nterm):
+            orc.thermal_iteration(o, lut, tctr[t, 0], tctr[t, 1], [g[2] for g in gens])
     H = pkg.host()
     sch = pkg.schedule(2.0, 0.7, maxtime, 2) if use_sched else pkg.schedule(0, 0, maxtime, -1, 0.90909090909090)
     import ctypes as C
     nref = H.msa_sched_count(C.byref(sch), maxtime)
     rt = [H.msa_sched_time(C.byref(sch), k) for k in range(nref)]
     rb = [H.msa_sched_beta(C.byref(sch), k) for k in range(nref)]
-    ctr, keys, _, luts = oracle_schedule(orc, nr, rt, rb, seed_mc=3003, jobid=3)
+    ctr, keys, _, luts = oracle_schedule(orc, nr, rt, rb, seed_mc=3003, jobid=3, gens=gens)
     want = {}
     for t in range(maxtime):
         if t in rt:

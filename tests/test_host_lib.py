@@ -126,6 +126,26 @@ def test_schedule_tables_match_oracle(pkg, orc, every, niter):
     assert list(idx) == list(range(nref))
 
 
+def test_thermal_counters_and_continuation_match_oracle(pkg, orc):
+    """Thermal counters (iteration 2t+p, parity p), then the refresh counters
+    continuing from the same generators, against the oracle's _fill_bits."""
+    from sched_util import oracle_schedule, oracle_thermal
+    R, nterm, jobid = 2, 4, 11
+    gens = pkg.generators([3003, 3004])
+    tctr = pkg.thermal_counters(gens, nterm, jobid)
+    ogens, octr = oracle_thermal(orc, R, nterm, seed_mc=3003, jobid=jobid)
+    assert np.array_equal(tctr, octr)
+    sch = pkg.schedule(2.0, 0.7, 20, 3)
+    ctr, keys, luts, idx = pkg.schedule_tables_from(sch, 20, gens, jobid)
+    H = pkg.host()
+    nref = ctr.shape[0]
+    times = [H.msa_sched_time(C.byref(sch), k) for k in range(nref)]
+    betas = [H.msa_sched_beta(C.byref(sch), k) for k in range(nref)]
+    rctr, rkeys, raw, _ = oracle_schedule(orc, R, times, betas, jobid=jobid, gens=ogens)
+    assert np.array_equal(ctr, rctr) and np.array_equal(keys, rkeys)
+    assert luts.tobytes() == raw
+
+
 def _declared(header):
     text = open(os.path.join(REPO, "include", header)).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
