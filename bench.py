@@ -260,6 +260,8 @@ def main():
     ap.add_argument("--served-load", type=float, default=0.8,
                     help="arrival rate as a fraction of the measured batched capacity")
     ap.add_argument("--autotune", action="store_true", help="autotune the sweep kernel first")
+    ap.add_argument("--thermal-iters", type=int, default=0,
+                    help="also time this many Metropolis thermalization iterations (msa_run_thermal)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the lattice is not sharded); "
@@ -291,6 +293,26 @@ def main():
     # the dominant kernel alone: sweeps without a refresh
     ksweeps = min(args.steps, 1024)
     ms_k = lat.run_timed(args.warmup + args.steps, ksweeps, 0)
+    # the state across the boundary: host arrays in and out (PCIe), once
+    p0 = time.perf_counter()
+    S_h, D0_h, D1_h = lat.download()
+    J_h = lat.couplings()
+    p1 = time.perf_counter()
+    lat.upload(J_h, S_h, D0_h, D1_h)
+    lat.sync()
+    p2 = time.perf_counter()
+    thermal = None
+    if args.thermal_iters > 0:
+        gens = pkg.generators([3003 + r for r in range(R)])
+        tctr = pkg.thermal_counters(gens, args.thermal_iters, 0)
+        lut = pkg.lut_beta(1.0 / 1.8)
+        lat.run_thermal(lut, tctr[:1], gens[2])             # warm-up
+        h0 = time.perf_counter()
+        lat.run_thermal(lut, tctr, gens[2])
+        h1 = time.perf_counter()
+        thermal = {"iterations": args.thermal_iters, "beta": round(1.0 / 1.8, 6),
+                   "ms_per_iteration": round((h1 - h0) * 1e3 / args.thermal_iters, 4),
+                   "spin_updates_per_ps": round(64.0 * R * V * args.thermal_iters / (h1 - h0) / 1e12, 4)}
     lat.close()
 
     t0 = args.warmup
@@ -340,10 +362,16 @@ def main():
             "timed_region_alg_GBs": round(alg_bytes(V, R, args.steps, nref) / sec / 1e9, 1),
         },
         "wall_s": round(wall, 4),
+        "state_transfer": {"bytes_each_way": int(S_h.nbytes + D0_h.nbytes + D1_h.nbytes + J_h.nbytes),
+                           "download_s": round(p1 - p0, 4), "upload_s": round(p2 - p1, 4),
+                           "note": "host arrays in canonical order, layout conversion included; "
+                                   "not part of value"},
         "mean_energy_per_site": round(float(-m["energy"].mean()) / V, 6),
     }
     if tuned is not None:
         out["autotune_ms_per_sweep"] = round(tuned[3], 5)
+    if thermal is not None:
+        out["thermal"] = thermal
     if args.batched > 0:
         out["batched"] = batched_vs_plain(pkg, args.batched, args.batched_len_div)
     if args.served > 0:

@@ -198,7 +198,11 @@ int msa_batch_progress(msa_batch *b, int32_t nslots, int64_t *t, int64_t *tend);
 int msa_batch_measure(msa_batch *b, int32_t slot, int32_t *unsat, int32_t *demon, int32_t *q,
                       int32_t *qlink);
 int msa_batch_download_state(msa_batch *b, int32_t slot, uint64_t *spin, uint64_t *d0, uint64_t *d1);
-/* Retire a lattice and free its memory. */
+/* Retire a lattice.  Its device buffers are kept (up to 32 lattices) and
+ * reused by a later admission of the same geometry, so a serving loop does no
+ * device allocation in steady state; lattices kept this way count against
+ * max_bytes together with those admitted, and are freed first when a new
+ * geometry needs room. */
 int msa_batch_remove(msa_batch *b, int32_t slot);
 
 /* HBM copy probe (no reference counterpart): copies `bytes` between two

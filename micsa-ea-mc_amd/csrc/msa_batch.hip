@@ -150,6 +150,17 @@ struct msa_batch {
   std::vector<msa_ctx *> slot;          // nullptr = free
   std::vector<int64_t> slot_bytes;
   std::vector<int32_t *> slot_lut_idx;  // device copies
+  std::vector<int32_t> slot_idx_cap;    // their capacity
+  // retired lattices kept for reuse by admissions of the same geometry
+  // (no allocation on the device); their bytes count against max_bytes
+  struct Idle {
+    msa_ctx *c;
+    int32_t *idx;
+    int32_t cap;
+    int64_t bytes;
+  };
+  std::vector<Idle> idle;
+  int64_t idle_bytes = 0;
   std::vector<BLat> h_lats;
   std::vector<char> fresh;              // added since the device table was last pushed
   BLat *d_lats = nullptr;
@@ -159,6 +170,7 @@ struct msa_batch {
   msa::FineLut *luts = nullptr;         // shared search tables, 2 per pair
   int32_t nlut = 0;
   int block = 256;
+  int ncu = 256;                        // compute units of the device
   int32_t chunk = 16;                   // iterations per captured graph
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -199,6 +211,8 @@ extern "C" int msa_batch_create(int device, int64_t max_bytes, msa_batch **out) 
   if (rc) return rc;
   msa_batch *b = new msa_batch();
   b->device = device;
+  if (hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || b->ncu < 1)
+    b->ncu = 256;
   b->max_bytes = max_bytes > 0 ? max_bytes : INT64_MAX;
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
     delete b;
@@ -216,6 +230,10 @@ extern "C" int msa_batch_destroy(msa_batch *b) {
   for (size_t s = 0; s < b->slot.size(); ++s) {
     if (b->slot[s]) msa_destroy(b->slot[s]);
     (void)hipFree(b->slot_lut_idx[s]);
+  }
+  for (auto &e : b->idle) {
+    msa_destroy(e.c);
+    (void)hipFree(e.idx);
   }
   (void)hipFree(b->d_lats);
   (void)hipFree(b->d_items);
@@ -271,39 +289,62 @@ extern "C" int msa_batch_add(msa_batch *b, const msa_job *j, int32_t *slot_out) 
   if (b->used_bytes + bytes > b->max_bytes)
     return msa_fail(MSA_ERR_NOMEM, "admission: %lld bytes in use + %lld > limit %lld",
                     (long long)b->used_bytes, (long long)bytes, (long long)b->max_bytes);
+  // an idle lattice of the same geometry, else make room among the idle ones;
+  // nothing of the batch may still run when an idle lattice's buffers are rewritten
+  HIPCHK(hipSetDevice(b->device));
+  HIPCHK(hipStreamSynchronize(b->stream));
   msa_ctx *c = nullptr;
-  int rc = msa_create(b->device, &j->geom, &c);
-  if (rc) return rc;
-  if ((rc = msa_init_philox(c, j->seed)) || (rc = msa_ctx_share_luts(c, b->luts, b->nlut)) ||
-      (j->nref > 0 && (rc = msa_set_refresh(c, j->nref, j->ctr, j->key, j->lut_idx)))) {
-    msa_destroy(c);
-    return rc;
-  }
   int32_t *d_idx = nullptr;
+  int32_t idx_cap = 0;
+  for (size_t k = 0; k < b->idle.size(); ++k) {
+    const msa_geom &g = b->idle[k].c->geom;
+    if (g.L == j->geom.L && g.R == j->geom.R && g.nc == j->geom.nc) {
+      c = b->idle[k].c;
+      d_idx = b->idle[k].idx;
+      idx_cap = b->idle[k].cap;
+      b->idle_bytes -= b->idle[k].bytes;
+      b->idle.erase(b->idle.begin() + (long)k);
+      break;
+    }
+  }
+  while (!c && !b->idle.empty() && b->used_bytes + b->idle_bytes + bytes > b->max_bytes) {
+    msa_destroy(b->idle.back().c);
+    (void)hipFree(b->idle.back().idx);
+    b->idle_bytes -= b->idle.back().bytes;
+    b->idle.pop_back();
+  }
+  int rc = MSA_OK;
+  if (!c && (rc = msa_create(b->device, &j->geom, &c))) return rc;
+  auto drop = [&](int code) {
+    msa_destroy(c);
+    (void)hipFree(d_idx);
+    return code;
+  };
+  if ((rc = msa_init_philox(c, j->seed)) || (rc = msa_ctx_share_luts(c, b->luts, b->nlut)) ||
+      (j->nref > 0 && (rc = msa_set_refresh(c, j->nref, j->ctr, j->key, j->lut_idx))))
+    return drop(rc);
   if (j->nref > 0) {
-    if (hipMalloc(&d_idx, sizeof(int32_t) * (size_t)j->nref) != hipSuccess) {
-      msa_destroy(c);
-      return msa_fail(MSA_ERR_NOMEM, "refresh index table");
+    if (j->nref > idx_cap) {
+      (void)hipFree(d_idx);
+      d_idx = nullptr;
+      idx_cap = 0;
+      if (hipMalloc(&d_idx, sizeof(int32_t) * (size_t)j->nref) != hipSuccess)
+        return drop(msa_fail(MSA_ERR_NOMEM, "refresh index table"));
+      idx_cap = j->nref;
     }
     if (hipMemcpyAsync(d_idx, j->lut_idx, sizeof(int32_t) * (size_t)j->nref, hipMemcpyHostToDevice,
                        b->stream) != hipSuccess ||
-        hipStreamSynchronize(b->stream) != hipSuccess) {
-      (void)hipFree(d_idx);
-      msa_destroy(c);
-      return msa_fail(MSA_ERR_HIP, "refresh index upload");
-    }
+        hipStreamSynchronize(b->stream) != hipSuccess)
+      return drop(msa_fail(MSA_ERR_HIP, "refresh index upload"));
   }
-  if ((rc = batch_pull(b))) {
-    (void)hipFree(d_idx);
-    msa_destroy(c);
-    return rc;
-  }
+  if ((rc = batch_pull(b))) return drop(rc);
   size_t s = 0;
   while (s < b->slot.size() && b->slot[s]) ++s;
   if (s == b->slot.size()) {
     b->slot.push_back(nullptr);
     b->slot_bytes.push_back(0);
     b->slot_lut_idx.push_back(nullptr);
+    b->slot_idx_cap.push_back(0);
     b->h_lats.push_back(BLat{});
     b->fresh.push_back(0);
   }
@@ -311,6 +352,7 @@ extern "C" int msa_batch_add(msa_batch *b, const msa_job *j, int32_t *slot_out) 
   b->slot[s] = c;
   b->slot_bytes[s] = bytes;
   b->slot_lut_idx[s] = d_idx;
+  b->slot_idx_cap[s] = idx_cap;
   b->used_bytes += bytes;
   BLat &L = b->h_lats[s];
   L = BLat{};
@@ -375,6 +417,8 @@ static int batch_prepare(msa_batch *b) {
   int rc = batch_pull(b);               // keep the progress of lattices already running
   if (rc) return rc;
   std::vector<BItem> items;
+  std::vector<uint32_t> tiles(n, 0);
+  uint64_t total = 0;
   for (size_t s = 0; s < n; ++s) {
     if (!b->slot[s]) continue;
     BLat &L = b->h_lats[s];
@@ -385,14 +429,21 @@ static int batch_prepare(msa_batch *b) {
     L.mLh = div_recip_of(Lh);
     L.mL = div_recip_of(L.g.L);
     L.done = 0;
-    // tiles per item: up to 16, so a block stages the search tables once for
-    // several tiles of a large lattice while small lattices keep one tile each
-    const uint32_t ntiles = ((rows + L.nr - 1) / L.nr) * L.nxc;
-    uint32_t per = ntiles / 16;
-    per = per < 1 ? 1 : (per > 16 ? 16 : per);
+    tiles[s] = ((rows + L.nr - 1) / L.nr) * L.nxc;
+    if (L.t < L.tend) total += tiles[s];
+  }
+  // tiles per item: a block stages the search tables once for several tiles,
+  // up to 16, while the unfinished lattices still give every CU about four
+  // items to balance over
+  const uint64_t want = 4ull * (uint64_t)b->ncu;
+  uint32_t per = (uint32_t)((total + want - 1) / want);
+  per = per < 1 ? 1 : (per > 16 ? 16 : per);
+  for (size_t s = 0; s < n; ++s) {
+    if (!b->slot[s]) continue;
+    BLat &L = b->h_lats[s];
     uint32_t ni = 0;
-    for (uint32_t t0 = 0; t0 < ntiles; t0 += per, ++ni)
-      if (L.t < L.tend) items.push_back(BItem{(uint32_t)s, t0, ntiles - t0 < per ? ntiles - t0 : per, 0});
+    for (uint32_t t0 = 0; t0 < tiles[s]; t0 += per, ++ni)
+      if (L.t < L.tend) items.push_back(BItem{(uint32_t)s, t0, tiles[s] - t0 < per ? tiles[s] - t0 : per, 0});
     L.nitems = ni;
   }
   if (n > b->cap) {
@@ -451,20 +502,38 @@ extern "C" int msa_batch_run(msa_batch *b, int64_t niter) {
   if (!b) return msa_fail(MSA_ERR_ARG, "NULL batch");
   if (niter < 0) return msa_fail(MSA_ERR_ARG, "niter=%lld", (long long)niter);
   HIPCHK(hipSetDevice(b->device));
-  if (b->dirty) {
-    HIPCHK(hipStreamSynchronize(b->stream));
-    int rc = batch_prepare(b);
-    if (rc) return rc;
-  }
-  if (!b->nitems) return MSA_OK;
+  // Every unfinished lattice advances one iteration per iteration, so the
+  // host knows where each reaches its length: the run is cut there and the
+  // work list rebuilt without the finished lattices (their blocks would only
+  // exit, and the remaining tiles are regrouped over the CUs).
   int64_t left = niter;
-  while (left >= b->chunk) {
-    HIPCHK(hipGraphLaunch(b->gexec, b->stream));
-    left -= b->chunk;
+  while (left > 0) {
+    if (b->dirty) {
+      HIPCHK(hipStreamSynchronize(b->stream));
+      int rc = batch_prepare(b);
+      if (rc) return rc;
+    }
+    if (!b->nitems) break;
+    int64_t step = left;
+    for (size_t s = 0; s < b->slot.size(); ++s)
+      if (b->slot[s] && b->h_lats[s].t < b->h_lats[s].tend)
+        step = std::min<int64_t>(step, (int64_t)(b->h_lats[s].tend - b->h_lats[s].t));
+    int64_t k = step;
+    while (k >= b->chunk) {
+      HIPCHK(hipGraphLaunch(b->gexec, b->stream));
+      k -= b->chunk;
+    }
+    for (int64_t it = 0; it < k; ++it)
+      for (int col = 0; col < 2; ++col) batch_launch(b, col);
+    HIPCHK(hipGetLastError());
+    for (size_t s = 0; s < b->slot.size(); ++s) {
+      BLat &L = b->h_lats[s];
+      if (!b->slot[s] || L.t >= L.tend) continue;
+      L.t += (uint64_t)step;
+      if (L.t >= L.tend) b->dirty = true;
+    }
+    left -= step;
   }
-  for (int64_t it = 0; it < left; ++it)
-    for (int col = 0; col < 2; ++col) batch_launch(b, col);
-  HIPCHK(hipGetLastError());
   return MSA_OK;
 }
 
@@ -522,8 +591,13 @@ extern "C" int msa_batch_remove(msa_batch *b, int32_t slot) {
   HIPCHK(hipSetDevice(b->device));
   int rc = batch_pull(b);               // keep the progress of the others
   if (rc) return rc;
-  msa_destroy(b->slot[slot]);
-  (void)hipFree(b->slot_lut_idx[slot]);
+  if (b->idle.size() < 32) {            // keep it for a later admission of its geometry
+    b->idle.push_back({b->slot[slot], b->slot_lut_idx[slot], b->slot_idx_cap[slot], b->slot_bytes[slot]});
+    b->idle_bytes += b->slot_bytes[slot];
+  } else {
+    msa_destroy(b->slot[slot]);
+    (void)hipFree(b->slot_lut_idx[slot]);
+  }
   b->slot[slot] = nullptr;
   b->slot_lut_idx[slot] = nullptr;
   b->used_bytes -= b->slot_bytes[slot];

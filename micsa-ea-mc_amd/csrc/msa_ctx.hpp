@@ -22,6 +22,7 @@ struct msa_ctx {
   uint4 *ctr = nullptr;
   uint32_t *key = nullptr;
   int32_t nref = 0;
+  int32_t ctr_cap = 0;                    // refreshes the ctr buffer holds
   std::vector<int32_t> lut_idx;           // host copy, indexes luts
   int32_t *meas = nullptr;                // [64]
   uint64_t *th_rnd = nullptr;             // thermal random words [3][V/128 * R]

@@ -404,8 +404,11 @@ __global__ void __launch_bounds__(R >= 4 ? 768 : 1024) k_sweep_fused2(FusedArgs 
   const uint32_t rm = yh ? (row == 0 ? 0 : row - 1) : (row == 0 ? L - 1 : row - 1);
   const uint32_t rp = yh ? (row + 1 == a.ny ? row : row + 1) : (row + 1 == L ? 0 : row + 1);
   const uint32_t ym = msa::wrapi((int)y - 1, L), yp = msa::wrapi((int)y + 1, L);
-  const uint32_t slot = (row * Lh + i) * R;              // this thread's LDS slot
-  const uint32_t plane = a.ny * Lh * R;                  // one LDS plane
+  // LDS planes hold one sub-plane per replica, so the 8-byte words that a
+  // wavefront reads for one replica are contiguous (no bank conflicts)
+  const uint32_t pr = a.ny * Lh;                         // one replica's sub-plane
+  const uint32_t slot = row * Lh + i;                    // this thread's LDS slot
+  const uint32_t plane = pr * R;                         // one LDS plane
   uint64_t *S1p = s_lds;                                 // odd spins, plane s
   uint64_t *Eb = s_lds + plane;                          // new even, planes s-1 / s
   int dx, dy, dz;
@@ -452,7 +455,7 @@ __global__ void __launch_bounds__(R >= 4 ? 768 : 1024) k_sweep_fused2(FusedArgs 
     const uint64_t j1x = ldo(Jx1, h * 8u), j1y = ldo(Jy1, h * 8u), j1z = ldo(Jz1, h * 8u);
     const uint64_t c0J3 = ldo(Jx1, (im + yb + LhL * z) * 8u), c0J4 = ldo(Jy1, (i + ymb + LhL * z) * 8u);
 #pragma unroll
-    for (int r = 0; r < R; ++r) S1p[slot + r] = S0w[r];
+    for (int r = 0; r < R; ++r) S1p[r * pr + slot] = S0w[r];
     __syncthreads();
     // ---- colour 0, plane s
     {
@@ -470,13 +473,13 @@ __global__ void __launch_bounds__(R >= 4 ? 768 : 1024) k_sweep_fused2(FusedArgs 
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         uint64_t D0 = dv[r].x, D1 = dv[r].y;
-        const uint64_t nxp = S1p[(row * Lh + ip) * R + r];
-        const uint64_t nxm = S1p[(row * Lh + im) * R + r];
-        const uint64_t nyp = yh && row + 1 == a.ny ? Hv[r] : S1p[(rp * Lh + i) * R + r];
-        const uint64_t nym = yh && row == 0 ? Hv[r] : S1p[(rm * Lh + i) * R + r];
+        const uint64_t nxp = S1p[r * pr + row * Lh + ip];
+        const uint64_t nxm = S1p[r * pr + row * Lh + im];
+        const uint64_t nyp = yh && row + 1 == a.ny ? Hv[r] : S1p[r * pr + rp * Lh + i];
+        const uint64_t nym = yh && row == 0 ? Hv[r] : S1p[r * pr + rm * Lh + i];
         E0[r] = msa::site_update(S0v[r], nxp, nyp, Sp[r], nxm, nym, Sm[r], j0x, j0y, j0z, c0J3,
                                  c0J4, j1z_prev, D0, D1);
-        Ew[slot + r] = E0[r];
+        Ew[r * pr + slot] = E0[r];
         if (write) {
           sto_stream(a.sp_out + (uint64_t)(2 * r) * H, h * 8u, E0[r]);
           sto_stream(Dout0 + r * rstride, hw * 16u, D0, D1);
@@ -496,10 +499,10 @@ __global__ void __launch_bounds__(R >= 4 ? 768 : 1024) k_sweep_fused2(FusedArgs 
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         uint64_t D0 = dv[r].x, D1 = dv[r].y;
-        const uint64_t nxp = Er[(row * Lh + ip) * R + r];
-        const uint64_t nxm = Er[(row * Lh + im) * R + r];
-        const uint64_t nyp = Er[(rp * Lh + i) * R + r];
-        const uint64_t nym = Er[(rm * Lh + i) * R + r];
+        const uint64_t nxp = Er[r * pr + row * Lh + ip];
+        const uint64_t nxm = Er[r * pr + row * Lh + im];
+        const uint64_t nyp = Er[r * pr + rp * Lh + i];
+        const uint64_t nym = Er[r * pr + rm * Lh + i];
         const uint64_t sn = msa::site_update(Sm[r], nxp, nyp, E0[r], nxm, nym, Em2[r], j1x_prev,
                                              j1y_prev, j1z_prev, c1J3, c1J4, j0z_m2, D0, D1);
         sto_stream(a.sp_out + (uint64_t)(2 * r + 1) * H, ho * 8u, sn);
@@ -938,15 +941,18 @@ extern "C" int msa_set_refresh(msa_ctx *c, int32_t nref, const msa_u32x4 *ctr,
                   c->nlut - 1);
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipStreamSynchronize(c->stream));
-  (void)hipFree(c->ctr);
-  (void)hipFree(c->key);
-  c->ctr = nullptr;
-  c->key = nullptr;
   c->nref = 0;
   const size_t R = c->g.R;
-  if (hipMalloc(&c->ctr, sizeof(uint4) * R * (size_t)nref) != hipSuccess ||
-      hipMalloc(&c->key, sizeof(uint32_t) * R) != hipSuccess)
-    return msa_fail(MSA_ERR_NOMEM, "refresh table of %d entries", nref);
+  if (nref > c->ctr_cap) {           // buffers are kept and reused when large enough
+    (void)hipFree(c->ctr);
+    c->ctr = nullptr;
+    c->ctr_cap = 0;
+    if (hipMalloc(&c->ctr, sizeof(uint4) * R * (size_t)nref) != hipSuccess)
+      return msa_fail(MSA_ERR_NOMEM, "refresh table of %d entries", nref);
+    c->ctr_cap = nref;
+  }
+  if (!c->key && hipMalloc(&c->key, sizeof(uint32_t) * R) != hipSuccess)
+    return msa_fail(MSA_ERR_NOMEM, "refresh keys");
   HIPCHK(hipMemcpyAsync(c->ctr, ctr, sizeof(uint4) * R * (size_t)nref, hipMemcpyHostToDevice,
                         c->stream));
   HIPCHK(hipMemcpyAsync(c->key, key, sizeof(uint32_t) * R, hipMemcpyHostToDevice, c->stream));

@@ -122,6 +122,7 @@ def serve(requests, max_bytes, epoch=64, block=None, tables=None, keep_state=Fal
     nxt = 0
     queue = []
     active = {}                 # slot -> (request, admit time)
+    prepared = {}               # rid -> job tables built while the device ran
     done = {}
     updates = 0
     max_active = 0
@@ -133,10 +134,12 @@ def serve(requests, max_bytes, epoch=64, block=None, tables=None, keep_state=Fal
             nxt += 1
         while queue:                              # FIFO admission by footprint
             req = queue[0]
-            slot = batch.add(tables.job(req, req.rid))
+            job = prepared.pop(req.rid, None) or tables.job(req, req.rid)
+            slot = batch.add(job)
             if slot is None:
                 if not active:
                     raise RuntimeError("request %d alone exceeds the memory limit" % req.rid)
+                prepared[req.rid] = job
                 break
             queue.pop(0)
             active[slot] = (req, clock() - t_start)
@@ -147,6 +150,10 @@ def serve(requests, max_bytes, epoch=64, block=None, tables=None, keep_state=Fal
                 time.sleep(max(0.0, min(0.01, reqs[nxt].arrival - (clock() - t_start))))
             continue
         batch.run(epoch)
+        # while the device runs the epoch: the tables of queued requests
+        for req in queue:
+            if req.rid not in prepared:
+                prepared[req.rid] = tables.job(req, req.rid)
         nslots = max(active) + 1
         t, tend = batch.progress(nslots)
         for slot in [s for s in active if t[s] >= tend[s]]:
