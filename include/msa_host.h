@@ -108,6 +108,16 @@ int msa_sched_tables(const msa_sched *s, int64_t nref, int R, const uint64_t *se
 int msa_sched_tables_from(const msa_sched *s, int64_t nref, int R, msa_prng *pr, msa_xo256 *xo,
                           uint32_t jobid, msa_u32x4 *ctr, msa_lut *luts, int32_t *lut_idx);
 
+/* The same for a run resumed at refresh k0 (k0 = msa_sched_count(s, now)):
+ * pr/xo hold the generators after the draws of refreshes 0..k0-1; counters of
+ * those refreshes are zero, as a run from iteration now never reads them. */
+int msa_sched_tables_resume(const msa_sched *s, int64_t k0, int64_t nref, int R, msa_prng *pr,
+                            msa_xo256 *xo, uint32_t jobid, msa_u32x4 *ctr, msa_lut *luts,
+                            int32_t *lut_idx);
+
+/* Advance each replica's generators by n _fill_bits draws. */
+void msa_gen_skip(msa_prng *pr, msa_xo256 *xo, int R, int64_t n);
+
 /* Counters of the canonical thermalization (Monte_Carlo with flag = 0,
  * main.cu:202-229): for t = 0 .. nterm-1 and parity p = 0, 1, _fill_bits at
  * iteration 2t + p with parity p for every replica, ctr[(t*2 + p)*R + r].
@@ -150,6 +160,79 @@ int  msa_read_schedule(const char *path, msa_sched *s, char *err, int errlen);
 /* Mean and error over replicas as Measures does (med.c:13-51):
  * mean of x_r, sqrt((<x^2> - <x>^2) / (n - 1)). */
 void msa_mean_err(const double *x, int n, double *mean, double *err);
+
+/* -------------------------------------------------------- checkpoints --
+ * The reference's checkpoint files (io.c:17-393) with NBETAS = 1 and NR = R
+ * replicas, byte for byte; L must be a multiple of 8 (its multispin layout).
+ * Functions return 0, or -1 with a message in err. */
+
+/* s_data as stored in conf files (header.h:98-111, seed_MC[16]): 184 bytes */
+typedef struct {
+  uint64_t maxtime, now;
+  int32_t ntimes, nterm, nc, nr, l, flag;
+  uint64_t seed_J, seed_u, seed_MC[MSA_MAXR];
+} msa_sdata;
+
+/* icds (header.h:69-78), the head of walkers files: 40 bytes */
+typedef struct {
+  uint32_t first, isample, nrep, walkers_per_site, lv, t;
+  uint64_t seed;
+  double beta;
+} msa_icds;
+
+#define MSA_PRNG_REF_BYTES 2080   /* sizeof(s_aleatorio_HQ_64bits), random_generator.h:29-36 */
+#define MSA_XO_REF_BYTES 48       /* sizeof(s_xoshiro256pp), random_generator.h:39-44 */
+
+/* Generator states as the reference's structs store them. */
+void msa_prng_to_ref(const msa_prng *g, uint8_t *out);
+void msa_prng_from_ref(const uint8_t *in, msa_prng *g);
+void msa_xo256_to_ref(const msa_xo256 *g, uint8_t *out);
+void msa_xo256_from_ref(const uint8_t *in, msa_xo256 *g);
+
+uint64_t msa_conf_bytes(int L);                      /* io.c:22-25 */
+uint64_t msa_walkers_bytes(int L, int R, int nc);    /* io.c:260-262 */
+uint64_t msa_msc_words(int L, int R, int nc);        /* demon words of one sample */
+
+/* write_conf (io.c:17-96) for one (sample, replica): <rep_dir>/conf.%016llu
+ * (now), hard link <rep_dir>/conf, link count and size checked.  janus: the
+ * V/8 spin bytes (msa_janus_from_lanes); energy = 3V - 2 * unsatisfied. */
+int msa_conf_write(const char *rep_dir, uint64_t now, const msa_sdata *d, double beta,
+                   const msa_prng *pr, const msa_xo256 *xo, int L, const uint8_t *janus,
+                   int32_t energy, char *err, int errlen);
+/* read_conf's parse of one file (io.c:110-196) after its size check. */
+int msa_conf_read(const char *path, int L, msa_sdata *d, double *beta, msa_prng *pr,
+                  msa_xo256 *xo, uint8_t *janus, int32_t *energy, char *err, int errlen);
+
+/* write_walkers (io.c:254-323): <sample_dir>/walkers.%016llu (now), hard link
+ * <sample_dir>/walkers; walkers.<prev> is removed when prev != 0.
+ * pr[R], xo[R]: the demon-refresh generators; w: msa_msc_words words. */
+int msa_walkers_write(const char *sample_dir, uint64_t now, uint64_t prev, const msa_icds *ic,
+                      int L, int R, int nc, const msa_prng *pr, const msa_xo256 *xo,
+                      const uint64_t *w, char *err, int errlen);
+/* read_walkers' parse of one file (io.c:325-393) after its size check. */
+int msa_walkers_read(const char *path, int L, int R, int nc, msa_icds *ic, msa_prng *pr,
+                     msa_xo256 *xo, uint64_t *w, char *err, int errlen);
+
+/* Spins of lanes lanes[0..n-1] in conf order (Janus_packing_for_write,
+ * io.c:212-231): janus[(i*R + r)*V/8 + (s >> 3)] bit s & 7.  _to_ overwrites
+ * only the listed lanes of spin[R][V]. */
+void msa_janus_from_lanes(int L, int R, const uint64_t *spin, int n, const int *lanes,
+                          uint8_t *janus);
+void msa_janus_to_lanes(int L, int R, const uint8_t *janus, int n, const int *lanes,
+                        uint64_t *spin);
+/* Demons d0/d1 [R][nc][V] of the listed lanes as the walkers file's words
+ * (packing_u placement, ini.c:301-345; word c*HALF*R + half*R + r of each
+ * colour, main.cu:868-872): w[i * msa_msc_words + ...], plane 0 colours 0, 1
+ * then plane 1 colours 0, 1.  _to_ overwrites only the listed lanes. */
+void msa_msc_from_lanes(int L, int R, int nc, const uint64_t *d0, const uint64_t *d1, int n,
+                        const int *lanes, uint64_t *w);
+void msa_msc_to_lanes(int L, int R, int nc, const uint64_t *w, int n, const int *lanes,
+                      uint64_t *d0, uint64_t *d1);
+
+/* calculate_scalar_energy (med.c:85-113) of every lane: e[r*64 + b]. */
+void msa_lane_energy(int L, int R, const uint64_t *J, const uint64_t *spin, int32_t *e);
+/* countJ of Init_J (ini.c:457-511): bonds with J bit 1, cnt[64]. */
+void msa_lane_countJ(int L, const uint64_t *J, int32_t *cnt);
 
 #ifdef __cplusplus
 }

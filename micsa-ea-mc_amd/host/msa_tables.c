@@ -100,18 +100,37 @@ double msa_sched_beta(const msa_sched *s, int64_t k)
   return 1.0 / T;
 }
 
-int msa_sched_tables_from(const msa_sched *s, int64_t nref, int R, msa_prng *pr, msa_xo256 *xo,
-                          uint32_t jobid, msa_u32x4 *ctr, msa_lut *luts, int32_t *lut_idx)
+int msa_sched_tables_resume(const msa_sched *s, int64_t k0, int64_t nref, int R, msa_prng *pr,
+                            msa_xo256 *xo, uint32_t jobid, msa_u32x4 *ctr, msa_lut *luts,
+                            int32_t *lut_idx)
 {
-  if (R < 1 || R > MSA_MAXR || nref < 0) return -1;
+  if (R < 1 || R > MSA_MAXR || nref < 0 || k0 < 0) return -1;
   for (int64_t k = 0; k < nref; k++) {
     const uint64_t t = (uint64_t)msa_sched_time(s, k);
-    for (int r = 0; r < R; r++) msa_fill_counter(&pr[r], &xo[r], t, 0, jobid, &ctr[k * R + r]);
+    for (int r = 0; r < R; r++) {
+      if (k >= k0)
+        msa_fill_counter(&pr[r], &xo[r], t, 0, jobid, &ctr[k * R + r]);
+      else
+        memset(&ctr[k * R + r], 0, sizeof(msa_u32x4));
+    }
     lut_idx[k] = (int32_t)k;
   }
 #pragma omp parallel for schedule(dynamic, 16)
   for (int64_t k = 0; k < nref; k++) msa_lut_demon(msa_sched_beta(s, k), &luts[2 * k], &luts[2 * k + 1]);
   return 0;
+}
+
+int msa_sched_tables_from(const msa_sched *s, int64_t nref, int R, msa_prng *pr, msa_xo256 *xo,
+                          uint32_t jobid, msa_u32x4 *ctr, msa_lut *luts, int32_t *lut_idx)
+{
+  return msa_sched_tables_resume(s, 0, nref, R, pr, xo, jobid, ctr, luts, lut_idx);
+}
+
+void msa_gen_skip(msa_prng *pr, msa_xo256 *xo, int R, int64_t n)
+{
+  msa_u32x4 c;
+  for (int64_t k = 0; k < n; k++)
+    for (int r = 0; r < R; r++) msa_fill_counter(&pr[r], &xo[r], 0, 0, 0, &c);
 }
 
 int msa_sched_tables(const msa_sched *s, int64_t nref, int R, const uint64_t *seed_MC,

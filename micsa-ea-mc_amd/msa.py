@@ -240,6 +240,21 @@ class Xo256(C.Structure):
     _fields_ = [("s", C.c_uint64 * 4), ("out", C.c_uint64)]
 
 
+class SData(C.Structure):
+    """s_data as stored in conf files (184 bytes)."""
+    _fields_ = [("maxtime", C.c_uint64), ("now", C.c_uint64), ("ntimes", C.c_int32),
+                ("nterm", C.c_int32), ("nc", C.c_int32), ("nr", C.c_int32), ("l", C.c_int32),
+                ("flag", C.c_int32), ("seed_J", C.c_uint64), ("seed_u", C.c_uint64),
+                ("seed_MC", C.c_uint64 * 16)]
+
+
+class Icds(C.Structure):
+    """icds, the head of walkers files (40 bytes)."""
+    _fields_ = [("first", C.c_uint32), ("isample", C.c_uint32), ("nrep", C.c_uint32),
+                ("walkers_per_site", C.c_uint32), ("lv", C.c_uint32), ("t", C.c_uint32),
+                ("seed", C.c_uint64), ("beta", C.c_double)]
+
+
 class Sched(C.Structure):
     _fields_ = [("T0", C.c_double), ("T1", C.c_double), ("nsweeps", C.c_int64),
                 ("refresh_every", C.c_int64), ("beta_fixed", C.c_double)]
@@ -283,6 +298,38 @@ def host():
                                             C.POINTER(Xo256), C.c_uint32, u32p, C.c_void_p, i32p]
         L.msa_sched_tables_grid.argtypes = [C.POINTER(Sched), C.c_int64, C.c_int, u64p, C.c_uint32,
                                             C.c_double, C.c_int32, C.c_int32, u32p, u32p, i32p]
+        L.msa_sched_tables_resume.argtypes = [C.POINTER(Sched), C.c_int64, C.c_int64, C.c_int,
+                                              C.POINTER(Prng), C.POINTER(Xo256), C.c_uint32, u32p,
+                                              C.c_void_p, i32p]
+        L.msa_gen_skip.argtypes = [C.POINTER(Prng), C.POINTER(Xo256), C.c_int, C.c_int64]
+        u8p = C.POINTER(C.c_uint8)
+        L.msa_conf_bytes.argtypes = [C.c_int]
+        L.msa_conf_bytes.restype = C.c_uint64
+        L.msa_walkers_bytes.argtypes = [C.c_int, C.c_int, C.c_int]
+        L.msa_walkers_bytes.restype = C.c_uint64
+        L.msa_msc_words.argtypes = [C.c_int, C.c_int, C.c_int]
+        L.msa_msc_words.restype = C.c_uint64
+        L.msa_prng_to_ref.argtypes = [C.POINTER(Prng), u8p]
+        L.msa_prng_from_ref.argtypes = [u8p, C.POINTER(Prng)]
+        L.msa_xo256_to_ref.argtypes = [C.POINTER(Xo256), u8p]
+        L.msa_xo256_from_ref.argtypes = [u8p, C.POINTER(Xo256)]
+        L.msa_conf_write.argtypes = [C.c_char_p, C.c_uint64, C.POINTER(SData), C.c_double,
+                                     C.POINTER(Prng), C.POINTER(Xo256), C.c_int, u8p, C.c_int32,
+                                     C.c_char_p, C.c_int]
+        L.msa_conf_read.argtypes = [C.c_char_p, C.c_int, C.POINTER(SData), C.POINTER(C.c_double),
+                                    C.POINTER(Prng), C.POINTER(Xo256), u8p, i32p, C.c_char_p,
+                                    C.c_int]
+        L.msa_walkers_write.argtypes = [C.c_char_p, C.c_uint64, C.c_uint64, C.POINTER(Icds), C.c_int,
+                                        C.c_int, C.c_int, C.POINTER(Prng), C.POINTER(Xo256), u64p,
+                                        C.c_char_p, C.c_int]
+        L.msa_walkers_read.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, C.POINTER(Icds),
+                                       C.POINTER(Prng), C.POINTER(Xo256), u64p, C.c_char_p, C.c_int]
+        L.msa_janus_from_lanes.argtypes = [C.c_int, C.c_int, u64p, C.c_int, i32p, u8p]
+        L.msa_janus_to_lanes.argtypes = [C.c_int, C.c_int, u8p, C.c_int, i32p, u64p]
+        L.msa_msc_from_lanes.argtypes = [C.c_int, C.c_int, C.c_int, u64p, u64p, C.c_int, i32p, u64p]
+        L.msa_msc_to_lanes.argtypes = [C.c_int, C.c_int, C.c_int, u64p, C.c_int, i32p, u64p, u64p]
+        L.msa_lane_energy.argtypes = [C.c_int, C.c_int, u64p, u64p, i32p]
+        L.msa_lane_countJ.argtypes = [C.c_int, u64p, i32p]
         _host = L
     return _host
 
@@ -481,3 +528,106 @@ class Batch:
     def remove(self, slot):
         _check(self.lib.msa_batch_remove(self.b, slot))
         self.geom.pop(slot, None)
+
+
+# ------------------------------------------------------------- checkpoints --
+_u8p = C.POINTER(C.c_uint8)
+
+
+def _lanes(lanes):
+    return np.ascontiguousarray(lanes, np.int32)
+
+
+def janus_from_lanes(L, spin, lanes):
+    """Conf spin payload (V/8 bytes per lane and replica): uint8 [n][R][V/8]."""
+    R, V = spin.shape
+    ln = _lanes(lanes)
+    out = np.zeros((len(ln), R, V // 8), np.uint8)
+    host().msa_janus_from_lanes(L, R, _p(np.ascontiguousarray(spin, np.uint64)), len(ln),
+                                _p(ln, i32p), _p(out, _u8p))
+    return out
+
+
+def janus_to_lanes(L, janus, lanes, spin):
+    """Write the conf payloads janus [n][R][V/8] into the listed lanes of spin [R][V] (in place)."""
+    R = spin.shape[0]
+    ln = _lanes(lanes)
+    host().msa_janus_to_lanes(L, R, _p(np.ascontiguousarray(janus, np.uint8), _u8p), len(ln),
+                              _p(ln, i32p), _p(spin))
+    return spin
+
+
+def msc_from_lanes(L, R, nc, d0, d1, lanes):
+    """Walkers-file demon words of the listed lanes: uint64 [n][msa_msc_words]."""
+    ln = _lanes(lanes)
+    nw = host().msa_msc_words(L, R, nc)
+    out = np.zeros((len(ln), nw), np.uint64)
+    host().msa_msc_from_lanes(L, R, nc, _p(np.ascontiguousarray(d0, np.uint64)),
+                              _p(np.ascontiguousarray(d1, np.uint64)), len(ln), _p(ln, i32p), _p(out))
+    return out
+
+
+def msc_to_lanes(L, R, nc, w, lanes, d0, d1):
+    ln = _lanes(lanes)
+    host().msa_msc_to_lanes(L, R, nc, _p(np.ascontiguousarray(w, np.uint64)), len(ln), _p(ln, i32p),
+                            _p(d0), _p(d1))
+    return d0, d1
+
+
+def lane_energy(L, J, spin):
+    """calculate_scalar_energy of every lane: int32 [R][64]."""
+    R = spin.shape[0]
+    e = np.zeros((R, 64), np.int32)
+    host().msa_lane_energy(L, R, _p(np.ascontiguousarray(J, np.uint64)),
+                           _p(np.ascontiguousarray(spin, np.uint64)), _p(e, i32p))
+    return e
+
+
+def lane_countJ(L, J):
+    c = np.zeros(64, np.int32)
+    host().msa_lane_countJ(L, _p(np.ascontiguousarray(J, np.uint64)), _p(c, i32p))
+    return c
+
+
+def _errbuf():
+    return C.create_string_buffer(512)
+
+
+def conf_write(rep_dir, now, sdata, beta, pr, xo, L, janus, energy):
+    e = _errbuf()
+    j = np.ascontiguousarray(janus, np.uint8)
+    if host().msa_conf_write(rep_dir.encode(), now, C.byref(sdata), beta, C.byref(pr), C.byref(xo),
+                             L, _p(j, _u8p), int(energy), e, len(e)):
+        raise MsaError(e.value.decode())
+
+
+def conf_read(path, L):
+    """(SData, beta, Prng, Xo256, janus uint8 [V/8], energy)."""
+    e = _errbuf()
+    d, beta, pr, xo, en = SData(), C.c_double(), Prng(), Xo256(), C.c_int32()
+    janus = np.zeros(L ** 3 // 8, np.uint8)
+    if host().msa_conf_read(path.encode(), L, C.byref(d), C.byref(beta), C.byref(pr), C.byref(xo),
+                            _p(janus, _u8p), C.byref(en), e, len(e)):
+        raise MsaError(e.value.decode())
+    return d, beta.value, pr, xo, janus, en.value
+
+
+def walkers_write(sample_dir, now, prev, icds, L, R, nc, prs, xos, w):
+    e = _errbuf()
+    PA, XA = Prng * R, Xo256 * R
+    pa, xa = PA(*prs), XA(*xos)
+    ww = np.ascontiguousarray(w, np.uint64)
+    if host().msa_walkers_write(sample_dir.encode(), now, prev, C.byref(icds), L, R, nc, pa, xa,
+                                _p(ww), e, len(e)):
+        raise MsaError(e.value.decode())
+
+
+def walkers_read(path, L, R, nc):
+    """(Icds, [Prng]*R, [Xo256]*R, uint64 words)."""
+    e = _errbuf()
+    ic = Icds()
+    pa, xa = (Prng * R)(), (Xo256 * R)()
+    w = np.zeros(host().msa_msc_words(L, R, nc), np.uint64)
+    if host().msa_walkers_read(path.encode(), L, R, nc, C.byref(ic), pa, xa, _p(w), e, len(e)):
+        raise MsaError(e.value.decode())
+    return ic, list(pa), list(xa), w

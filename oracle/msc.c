@@ -208,6 +208,18 @@ void orc_msc_unpack(const orc_msc *m, uint8_t *spin, uint8_t *d0, uint8_t *d1)
       }
 }
 
+/* The words a walkers file holds (copy_from_GPU_for_write main.cu:479-487,
+ * write_walkers io.c:276-288): walker0 colour 0, colour 1, then walker1
+ * colour 0, colour 1, each nc*HALF*R words. */
+void orc_msc_walker_words(const orc_msc *m, uint64_t *out)
+{
+  const size_t n = (size_t)m->nc * m->HALF * m->R;
+  memcpy(out, m->w0[0], n * sizeof(uint64_t));
+  memcpy(out + n, m->w0[1], n * sizeof(uint64_t));
+  memcpy(out + 2 * n, m->w1[0], n * sizeof(uint64_t));
+  memcpy(out + 3 * n, m->w1[1], n * sizeof(uint64_t));
+}
+
 /* d_oneMCstepBN_multisample, ISI branch (main.cu:847-1080), one colour */
 static void msc_mc(orc_msc *m, int dir, int ic)
 {

@@ -158,6 +158,7 @@ void     orc_msc_destroy(orc_msc *m);
 void     orc_msc_pack(orc_msc *m, const uint8_t *spin, const uint8_t *jx, const uint8_t *jy,
                       const uint8_t *jz, const uint8_t *d0, const uint8_t *d1);
 void     orc_msc_unpack(const orc_msc *m, uint8_t *spin, uint8_t *d0, uint8_t *d1);
+void     orc_msc_walker_words(const orc_msc *m, uint64_t *out /*[4*nc*HALF*R]*/);
 void     orc_msc_monte_carlo(orc_msc *m);       /* main.cu:254-306 (micro branch) */
 void     orc_msc_thermal(orc_msc *m, const orc_lut *lut, const uint32_t *ctr0,
                          const uint32_t *ctr1, const uint32_t *key);   /* main.cu:202-252 */

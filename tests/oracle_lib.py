@@ -138,6 +138,7 @@ class Oracle:
         L.orc_msc_destroy.argtypes = [C.c_void_p]
         L.orc_msc_pack.argtypes = [C.c_void_p] + [u8p] * 6
         L.orc_msc_unpack.argtypes = [C.c_void_p, u8p, u8p, u8p]
+        L.orc_msc_walker_words.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
         L.orc_msc_monte_carlo.argtypes = [C.c_void_p]
         L.orc_msc_energy.argtypes = [C.c_void_p, i32p]
         L.orc_msc_scalar_energy.argtypes = [C.c_int] + [u8p] * 4

@@ -40,7 +40,9 @@ def run_driver(d, extra=(), sample=3, nbits=2):
 @pytest.mark.parametrize("change,msg", [
     (dict(nterm=5, L=12), "multiple of 8"),
     (dict(nterm=-1), "Bad data input"),
-    (dict(flag=0), "only new runs"),
+    (dict(flag=0), "impossible to do a backup"),
+    (dict(flag=-1), "not recognize"),
+    (dict(flag=0, L=12), "Checkpoints need L to be a multiple of 8"),
     (dict(beta="0.5"), "betas[0]"),
     (dict(nc=7), "nc=7"),
 ])

@@ -1,0 +1,264 @@
+"""Checkpoint/restart of the driver bin/msa_anneal: flags 0/1/2, seeds files,
+the `running` lock, integrity checks on reading confs (io.c:98-210, 680-928,
+1016-1047), and -- on the GPU -- that a run stopped by its wall-clock budget,
+by SIGTERM or by its input's maxtime and restarted continues bit for bit as
+the uninterrupted run: same evol_*.txt, same conf and walkers files."""
+import ctypes as C
+import os
+import signal
+import subprocess
+import time
+
+import numpy as np
+import pytest
+
+from test_driver import DRIVER, run_driver, write_inputs
+
+BETA = "0.90909090909090"
+SEEDS = (1001, 2002, 3003, 3004)
+
+
+def sample_dir(out, sample=3):
+    return os.path.join(out, "I%03d" % sample)
+
+
+# ------------------------------------------------------------- CPU checks --
+def make_checkpoint(pkg, out, L=8, R=2, nc=1, lanes=(0, 1), now=4, sample=3):
+    """A consistent set of seeds, conf and walkers files for the driver's
+    flag 0, written with the host library: state = the run's initial state."""
+    H = pkg.host()
+    seed_J, seed_u, mc0 = SEEDS[0], SEEDS[1], SEEDS[2]
+    seed_mc = [mc0 + r for r in range(R)]
+    J, S = pkg.build_state(L, R, seed_J, seed_u)
+    V = L ** 3
+    D0 = np.zeros((R, nc, V), np.uint64)
+    D1 = np.zeros((R, nc, V), np.uint64)
+    sj = np.zeros(64, np.uint64)
+    H.msa_seeds_from_one(seed_J, sj.ctypes.data_as(C.POINTER(C.c_uint64)), 64)
+    sc = np.zeros(64, np.uint64)
+    H.msa_seeds_from_one(seed_u, sc.ctypes.data_as(C.POINTER(C.c_uint64)), 64)
+    cnt = pkg.lane_countJ(L, J)
+    en = pkg.lane_energy(L, J, S)
+    jan = pkg.janus_from_lanes(L, S, list(lanes))
+    w = pkg.msc_from_lanes(L, R, nc, D0, D1, list(lanes))
+    gens = []
+    for r in range(R):
+        pr, xo, key = pkg.Prng(), pkg.Xo256(), C.c_uint32()
+        H.msa_cpu_generators(C.byref(pr), C.byref(xo), C.byref(key), seed_mc[r])
+        gens.append((pr, xo))
+    sd = sample_dir(out, sample)
+    os.makedirs(sd, exist_ok=True)
+    with open(os.path.join(sd, "master_seedJ.in"), "w") as f:
+        f.write("%d\n" % seed_J)
+    for i, b in enumerate(lanes):
+        bd = os.path.join(sd, "BIT%03d" % b)
+        os.makedirs(bd, exist_ok=True)
+        with open(os.path.join(bd, "seeds.in"), "w") as f:
+            f.write("%d %d %s %d\n" % (int(sj[b]), seed_u, " ".join(map(str, seed_mc)), int(cnt[b])))
+        d = pkg.SData()
+        d.maxtime, d.now, d.ntimes, d.nterm, d.nc, d.nr, d.l, d.flag = 16, now, 5, 0, nc, R, L, 1
+        d.seed_J, d.seed_u = int(sj[b]), seed_u
+        for r in range(R):
+            d.seed_MC[r] = seed_mc[r]
+        for r in range(R):
+            rd = os.path.join(bd, "R%02d" % r)
+            os.makedirs(rd, exist_ok=True)
+            pkg.conf_write(rd, now, d, float(BETA), gens[r][0], gens[r][1], L, jan[i, r], en[r, b])
+        ic = pkg.Icds(0, i, R, nc, L, 0, int(sc[b]), float(BETA))
+        pkg.walkers_write(bd, now, 0, ic, L, R, nc, [g[0] for g in gens], [g[1] for g in gens], w[i])
+    return sd
+
+
+def test_lock_refuses_a_second_run(tmp_path):
+    d = str(tmp_path)
+    out = write_inputs(d)
+    sd = sample_dir(out)
+    os.makedirs(sd)
+    lock = os.path.join(sd, "running")
+    open(lock, "w").write("other 1 0 now\n")
+    p = run_driver(d)
+    assert p.returncode == 1 and "running exists" in p.stderr
+    assert os.path.exists(lock)                       # someone else's lock stays
+
+
+def test_new_run_refuses_existing_confs(tmp_path, pkg):
+    d = str(tmp_path)
+    out = write_inputs(d, flag=1)
+    make_checkpoint(pkg, out)
+    p = run_driver(d)
+    assert p.returncode == 1 and "There are some confs" in p.stderr
+    assert "There are some confs" in open(os.path.join(sample_dir(out), "error")).read()
+
+
+def test_missing_replica_conf(tmp_path, pkg):
+    d = str(tmp_path)
+    out = write_inputs(d, flag=0)
+    sd = make_checkpoint(pkg, out)
+    os.remove(os.path.join(sd, "BIT001", "R01", "conf"))
+    p = run_driver(d)
+    assert p.returncode == 1 and "less conf files" in p.stderr
+
+
+def test_seed_file_disagrees_with_input(tmp_path, pkg):
+    d = str(tmp_path)
+    out = write_inputs(d, flag=0, seeds=(1002,) + SEEDS[1:])
+    make_checkpoint(pkg, out)
+    p = run_driver(d)
+    assert p.returncode == 1 and "Master seedJ in input file" in p.stderr
+
+
+def test_good_checkpoint_passes_the_checks(tmp_path, pkg):
+    """Every check on the files passes; a CPU-only container then stops at
+    the first device call, so nothing before it complained."""
+    d = str(tmp_path)
+    out = write_inputs(d, flag=2)
+    make_checkpoint(pkg, out)
+    p = run_driver(d)
+    if p.returncode == 0:
+        pytest.skip("a GPU is present: the restart ran")
+    assert "msa_create" in p.stderr, p.stderr
+    assert not os.path.exists(os.path.join(sample_dir(out), "running"))
+
+
+def test_corrupt_conf_energy(tmp_path, pkg):
+    d = str(tmp_path)
+    out = write_inputs(d, flag=0)
+    sd = make_checkpoint(pkg, out)
+    path = os.path.join(sd, "BIT001", "R01", "conf")
+    raw = bytearray(open(path, "rb").read())
+    raw[2321 + 10] ^= 0x04                             # one spin of the payload
+    os.remove(path)
+    open(path, "wb").write(raw)
+    p = run_driver(d)
+    assert p.returncode == 1 and "Energy read and calculate differ in ibit=1 ir=1" in p.stderr
+
+
+def test_truncated_walkers(tmp_path, pkg):
+    d = str(tmp_path)
+    out = write_inputs(d, flag=0)
+    sd = make_checkpoint(pkg, out)
+    path = os.path.join(sd, "BIT000", "walkers")
+    raw = open(path, "rb").read()
+    os.remove(path)
+    open(path, "wb").write(raw[:-8])
+    p = run_driver(d)
+    assert p.returncode == 1 and "bytes in" in p.stderr
+
+
+def test_countJ_mismatch(tmp_path, pkg):
+    d = str(tmp_path)
+    out = write_inputs(d, flag=0)
+    sd = make_checkpoint(pkg, out)
+    path = os.path.join(sd, "BIT000", "seeds.in")
+    f = open(path).read().split()
+    f[-1] = str(int(f[-1]) + 1)
+    open(path, "w").write(" ".join(f) + "\n")
+    p = run_driver(d)
+    assert p.returncode == 1 and "countJ and countJ0 differ" in p.stderr
+
+
+# ------------------------------------------------------------- GPU runs --
+EVOL = ("evol_E_spins.txt", "evol_E_walkers.txt", "evol_T_walkers.txt", "evol_E_int.txt",
+        "evol_overlap.txt")
+
+
+def assert_same_outputs(outA, outB, lanes, R, sample=3):
+    for b in lanes:
+        ba = os.path.join(sample_dir(outA, sample), "BIT%03d" % b)
+        bb = os.path.join(sample_dir(outB, sample), "BIT%03d" % b)
+        for name in EVOL:
+            ta, tb = open(os.path.join(ba, name)).read(), open(os.path.join(bb, name)).read()
+            assert ta == tb, (b, name)
+        wa = open(os.path.join(ba, "walkers"), "rb").read()
+        assert wa == open(os.path.join(bb, "walkers"), "rb").read(), b
+        for r in range(R):
+            ca = open(os.path.join(ba, "R%02d" % r, "conf"), "rb").read()
+            cb = open(os.path.join(bb, "R%02d" % r, "conf"), "rb").read()
+            # s_data.flag (bytes 36..39) is 1 in a new run and 0 after a restart
+            assert ca[:36] == cb[:36] and ca[40:] == cb[40:], (b, r)
+            assert np.frombuffer(cb[36:40], np.int32)[0] == 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nterm,sched", [(0, None), (3, None), (0, "2.0\n0.7\n256\n4\n")])
+def test_restart_after_maxtime_continues_bit_for_bit(tmp_path, nterm, sched):
+    L, R, nc = 16, 2, 2
+    times = tuple(2 ** k for k in range(9))            # 1 .. 256
+    extra = []
+
+    def inputs(d, maxtime, flag):
+        out = write_inputs(d, L=L, nr=R, nc=nc, maxtime=maxtime, times=times, nterm=nterm,
+                           flag=flag)
+        if sched:
+            with open(os.path.join(d, "sched.in"), "w") as f:
+                f.write(sched)
+            extra[:] = ["0", "", os.path.join(d, "sched.in")]
+        return out
+
+    dA, dB = str(tmp_path / "A"), str(tmp_path / "B")
+    outA = inputs(dA, 256, 1)
+    p = run_driver(dA, extra)
+    assert p.returncode == 0, p.stderr
+    outB = inputs(dB, 32, 1)
+    p = run_driver(dB, extra)
+    assert p.returncode == 0 and "finished at iteration 32" in p.stdout, p.stderr
+    inputs(dB, 256, 2)
+    p = run_driver(dB, extra)
+    assert p.returncode == 0 and "finished at iteration 256" in p.stdout, p.stderr
+    if nterm:
+        assert "thermalization is not repeated" in p.stdout
+    assert_same_outputs(outA, outB, (0, 1), R)
+    assert not os.path.exists(os.path.join(sample_dir(outB), "running"))
+    assert len(open(os.path.join(sample_dir(outB), "run.log")).read().splitlines()) == 2
+
+
+LONG = dict(L=32, nr=2, nc=1, maxtime=2 ** 20, times=tuple(2 ** k for k in range(21)))
+
+
+@pytest.fixture(scope="module")
+def long_reference(tmp_path_factory):
+    d = str(tmp_path_factory.mktemp("long_ref"))
+    out = write_inputs(d, **LONG)
+    t0 = time.time()
+    p = run_driver(d)
+    assert p.returncode == 0, p.stderr
+    print("uninterrupted run: %.1f s" % (time.time() - t0), flush=True)
+    return out
+
+
+@pytest.mark.gpu
+def test_wall_budget_then_restart(tmp_path, long_reference):
+    d = str(tmp_path)
+    out = write_inputs(d, **LONG)
+    p = run_driver(d, ["1"])                          # max_time = 1 s
+    assert p.returncode == 0, p.stderr
+    stop = int(p.stdout.split("finished at iteration ")[1].split()[0])
+    assert stop < LONG["maxtime"] and stop in LONG["times"], stop
+    write_inputs(d, flag=2, **LONG)
+    p = run_driver(d)
+    assert p.returncode == 0 and "finished at iteration %d" % LONG["maxtime"] in p.stdout
+    assert_same_outputs(long_reference, out, (0, 1), LONG["nr"])
+
+
+@pytest.mark.gpu
+def test_sigterm_then_restart(tmp_path, long_reference):
+    d = str(tmp_path)
+    out = write_inputs(d, **LONG)
+    args = [DRIVER, "3", "2", os.path.join(d, "beta.in"), os.path.join(d, "input.in"),
+            os.path.join(d, "lut.bin"), os.path.join(d, "times.in"), "0"]
+    proc = subprocess.Popen(args, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+    mark = os.path.join(sample_dir(out), "BIT000", "R00", "conf.%016d" % 1024)
+    t0 = time.time()
+    while not os.path.exists(mark) and proc.poll() is None and time.time() - t0 < 60:
+        time.sleep(0.01)
+    assert proc.poll() is None, "the run ended before the signal"
+    proc.send_signal(signal.SIGTERM)
+    so, se = proc.communicate(timeout=60)
+    assert proc.returncode == 1 and "Received SIGTERM" in se, se
+    sd = sample_dir(out)
+    assert "Received SIGTERM" in open(os.path.join(sd, "error")).read()
+    assert not os.path.exists(os.path.join(sd, "running"))
+    write_inputs(d, flag=2, **LONG)
+    p = run_driver(d)
+    assert p.returncode == 0 and "finished at iteration %d" % LONG["maxtime"] in p.stdout, p.stderr
+    assert_same_outputs(long_reference, out, (0, 1), LONG["nr"])
