@@ -9,10 +9,13 @@ State is constructed on the device (counter-based) and stays in HBM; the
 timed region covers exactly --steps sweeps, bracketed by stream syncs and
 timed with HIP events on the library's stream.
 
+The sweep kernel and its tile are first autotuned on this lattice (every
+candidate gives bit-identical results; --no-autotune keeps the default).
 Prints one JSON line.  `value` = spin updates per picosecond
 (64 lanes x R x V x steps / seconds / 1e12).  `roofline` is for the
 dominant kernel, the sweep kernel of sweeps without a refresh: its
-algorithmic HBM bytes per launch, (48R+24) V, over its average launch time,
+algorithmic HBM bytes per launch, (48R+24) V per sweep times the sweeps one
+launch performs (two for the sweep-pair kernel), over its average launch time,
 measured with HIP events on the library's stream over a separate run of
 sweeps that all use it, against the MI355X HBM peak of 8 TB/s.  `traffic`
 (HBM bytes per launch from counters) is read from the profile summary in
@@ -45,7 +48,7 @@ def load_package():
     return mod
 
 
-KERNELS = {0: "k_halfsweep", 1: "k_sweep_fused<%d>", 2: "k_sweep_fused2<%d>"}
+KERNELS = {0: "k_halfsweep", 1: "k_sweep_fused<%d>", 2: "k_sweep_fused2<%d>", 3: "k_sweep_pair"}
 
 
 def kernel_name(mode, R):
@@ -259,7 +262,10 @@ def main():
     ap.add_argument("--served", type=int, default=60, help="requests in the served test (0: skip)")
     ap.add_argument("--served-load", type=float, default=0.8,
                     help="arrival rate as a fraction of the measured batched capacity")
-    ap.add_argument("--autotune", action="store_true", help="autotune the sweep kernel first")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the library's default tiling instead of autotuning it on this lattice")
+    ap.add_argument("--tiling", type=str, default="",
+                    help="MODE,TY,TZ: use this sweep kernel and tile (no autotune)")
     ap.add_argument("--thermal-iters", type=int, default=0,
                     help="also time this many Metropolis thermalization iterations (msa_run_thermal)")
     args = ap.parse_args()
@@ -274,8 +280,10 @@ def main():
     lat = pkg.Lattice(L, R, nc, device=0)
     lat.init_philox(0x5EED)
     tuned = None
-    if args.autotune:
-        tuned = lat.autotune(8)
+    if args.tiling:
+        lat.set_tiling(*[int(x) for x in args.tiling.split(",")])
+    elif not args.no_autotune:
+        tuned = lat.autotune(16)
     tiling = lat.tiling()
     total = args.warmup + args.steps
     sched = pkg.schedule(1.8, 0.6, nsched, every)
@@ -290,8 +298,9 @@ def main():
     lat.sync()
     wall = time.perf_counter() - w0
     m = lat.measure()
-    # the dominant kernel alone: sweeps without a refresh
-    ksweeps = min(args.steps, 1024)
+    # the dominant kernel alone: sweeps without a refresh (an even count, so
+    # that the sweep-pair kernel runs alone)
+    ksweeps = max(2, min(args.steps, 1024) & ~1)
     ms_k = lat.run_timed(args.warmup + args.steps, ksweeps, 0)
     # the state across the boundary: host arrays in and out (PCIe), once
     p0 = time.perf_counter()
@@ -320,8 +329,8 @@ def main():
     sec = ms * 1e-3
     updates = 64.0 * R * V * args.steps
     value = updates / sec / 1e12
-    per_sweep = (2 if tiling[0] == 0 else 1) * nc
-    launches_k = per_sweep * ksweeps
+    per_sweep = {0: 2.0, 1: 1.0, 2: 1.0, 3: 0.5}[tiling[0]] * nc
+    launches_k = int(per_sweep * ksweeps)
     kbytes = alg_bytes(V, R, ksweeps, 0) / launches_k
     avg_s = ms_k * 1e-3 / launches_k
     achieved = kbytes / avg_s / 1e9
@@ -345,7 +354,8 @@ def main():
                         "refresh every %d sweeps" % (L, R, nc, every),
             "L": L, "R": R, "nc": nc, "lanes": 64, "refresh_every": every,
             "sweeps_timed": args.steps, "refreshes_timed": nref,
-            "kernel_mode": ["two-pass", "one-pass", "one-pass, single-load"][tiling[0]],
+            "kernel_mode": ["two-pass", "one-pass", "one-pass, single-load",
+                            "two sweeps per pass"][tiling[0]],
             "tile_rows_planes": [tiling[1], tiling[2]],
         },
         "roofline": {

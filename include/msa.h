@@ -141,7 +141,11 @@ int msa_measure(msa_ctx *ctx, int32_t *unsat, int32_t *demon, int32_t *q, int32_
  * one per colour.  mode 1: one kernel per sweep whose blocks own ty rows x
  * tz planes (ty, tz dividing L; 0,0 = built-in default).  mode 2: the same
  * tiles, each word loaded once per block (register plane window, LDS
- * exchange) and tiles placed per XCD.  Sweeps with a refresh always use the
+ * exchange) and tiles placed per XCD.  mode 3 (nc = 1 only): two
+ * refresh-free iterations per kernel (temporal blocking) on ty x tz tiles
+ * (ty + 6 thread rows, one replica per block); a single refresh-free
+ * iteration left between refreshes uses the mode 2 kernel (or mode 0 where
+ * mode 2 does not exist for R).  Sweeps with a refresh always use the
  * two-kernel form.  Results are identical in every mode; only the speed
  * differs. */
 int msa_set_tiling(msa_ctx *ctx, int32_t mode, int32_t ty, int32_t tz);

@@ -30,6 +30,8 @@ struct msa_ctx {
   int fused = 0;                          // sweeps without refresh: 0 two-pass, 1/2 one-pass kernel
   uint32_t fty = 0, ftz = 0, fny = 0;     // its tile: rows, planes, thread rows
   int fyhalo = 1;
+  int pair = 0;                           // refresh-free sweeps two per pass (k_sweep_pair)
+  uint32_t pty = 0, ptz = 0;              // its tile: rows, planes
 };
 
 

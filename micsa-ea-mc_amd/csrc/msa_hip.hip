@@ -525,6 +525,199 @@ __global__ void __launch_bounds__(R >= 4 ? 768 : 1024) k_sweep_fused2(FusedArgs 
   }
 }
 
+struct PairArgs {
+  const uint64_t *sp_in;
+  uint64_t *sp_out;
+  const uint64_t *jp;
+  const ulonglong2 *dm_in;
+  ulonglong2 *dm_out;
+  Geo g;
+  uint32_t ty, tz;   // own rows, own planes
+  uint32_t ny;       // thread rows: ty + 6
+};
+
+// Block b of the sweep-pair kernel -> (row tile, plane slab, replica).  XCD x
+// (blocks b = x mod 8) gets the x-th contiguous run of (tile, replica) pairs,
+// the replicas of a tile next to each other, so the blocks that share halo
+// rows and J words run on one XCD at the same time.
+__device__ __forceinline__ void tile_of_pair(uint32_t b, uint32_t nty, uint32_t ntiles, uint32_t R,
+                                             uint32_t &tyi, uint32_t &tzi, uint32_t &r) {
+  const uint32_t n = ntiles * R;
+  const uint32_t t = (n % 8u) == 0 ? (b % 8u) * (n / 8u) + b / 8u : b;
+  r = t % R;
+  const uint32_t tile = t / R;
+  tyi = tile % nty;
+  tzi = tile / nty;
+}
+
+// Two consecutive sweeps of one replica, demon layer 0 (nc = 1), in one pass
+// (temporal blocking): the lattice state is read and written once per two
+// sweeps.  The z wavefront of k_sweep_fused2 carries four phases, step k at
+// plane s = z0 - 3 + k:
+//   A0  first sweep, colour 0, plane s      (all ny rows)
+//   A1  first sweep, colour 1, plane s - 1  (rows 1 .. ny-2)
+//   B0  second sweep, colour 0, plane s - 2 (rows 2 .. ny-3)
+//   B1  second sweep, colour 1, plane s - 3 (rows 3 .. ny-4: the tile's own rows)
+// Each phase needs the previous phase's values of its plane (in-plane
+// neighbours, from LDS, written one step earlier) and of the planes above and
+// below (the thread's own column, in registers).  Only the second sweep's
+// results of the own rows and planes are written.  The first sweep's demons
+// walk (layer 0: colour 0 by +1, colour 1 by -1 in the split index) to the
+// neighbouring thread of the same row through LDS two steps after they are
+// produced.  Every LDS buffer is double-buffered, so one barrier per step
+// orders all writers and readers.  J of the second sweep's planes is read
+// again (the first sweep loaded it two steps earlier; the lines are in L2).
+__global__ void __launch_bounds__(1024) k_sweep_pair(PairArgs a) {
+  using msa::ldo;
+  using msa::ldo_stream;
+  using msa::sto_stream;
+  using msa::wrapi;
+  extern __shared__ uint64_t s_lds[];
+  const Geo g = a.g;
+  const uint32_t Lh = g.Lh, L = g.L, ny = a.ny;
+  const uint64_t H = g.H;
+  const uint32_t i = threadIdx.x % Lh, row = threadIdx.x / Lh;
+  uint32_t tyi, tzi, r;
+  tile_of_pair(blockIdx.x, L / a.ty, (L / a.ty) * (L / a.tz), g.R, tyi, tzi, r);
+  const uint32_t y0 = tyi * a.ty, z0 = tzi * a.tz;
+  const uint32_t y = wrapi((int)(y0 + row) - 3, L);
+  const uint32_t ym = wrapi((int)y - 1, L), yp = wrapi((int)y + 1, L);
+  const uint32_t rm = row == 0 ? 0 : row - 1, rp = row + 1 == ny ? row : row + 1;
+  const bool rowA1 = row >= 1 && row + 1 < ny;
+  const bool rowB0 = row >= 2 && row + 2 < ny;
+  const bool rowB1 = row >= 3 && row + 3 < ny;
+  const bool hrow = row == 0 || row + 1 == ny;
+  const uint32_t P = ny * Lh, slot = row * Lh + i;
+  uint64_t *S1p = s_lds, *A0b = s_lds + 2 * P, *A1b = s_lds + 4 * P, *B0b = s_lds + 6 * P;
+  ulonglong2 *XD0 = reinterpret_cast<ulonglong2 *>(s_lds + 8 * P), *XD1 = XD0 + 2 * P;
+  const uint64_t *S0in = a.sp_in + (uint64_t)(2 * r) * H, *S1in = S0in + H;
+  uint64_t *S0out = a.sp_out + (uint64_t)(2 * r) * H, *S1out = S0out + H;
+  const ulonglong2 *Din0 = a.dm_in + (uint64_t)(2 * r) * H, *Din1 = Din0 + H;
+  ulonglong2 *Dout0 = a.dm_out + (uint64_t)(2 * r) * H, *Dout1 = Dout0 + H;
+  const uint64_t *Jx0 = a.jp, *Jy0 = a.jp + H, *Jz0 = a.jp + 2 * H;
+  const uint64_t *Jx1 = a.jp + 3 * H, *Jy1 = a.jp + 4 * H, *Jz1 = a.jp + 5 * H;
+  const uint32_t LhL = Lh * L;
+  const uint32_t yb = Lh * y, ymb = Lh * ym, hrb = row == 0 ? Lh * ym : Lh * yp;
+  const uint32_t iw0 = i + 1 == Lh ? 0 : i + 1, iw1 = i == 0 ? Lh - 1 : i - 1;
+
+  // odd input spins of the own column, planes s-1, s, s+1
+  uint64_t Sm = ldo(S1in, (i + yb + LhL * wrapi((int)z0 - 4, L)) * 8u);
+  uint64_t S0w = ldo(S1in, (i + yb + LhL * wrapi((int)z0 - 3, L)) * 8u);
+  uint64_t Sp = 0;
+  uint64_t E0 = 0, Em1 = 0, Em2 = 0;      // A0 results, planes s, s-1, s-2
+  uint64_t F0 = 0, F1 = 0, F2 = 0;        // A1 results, planes s-1, s-2, s-3
+  uint64_t G0 = 0, G1 = 0, G2 = 0;        // B0 results, planes s-2, s-3, s-4
+  uint64_t a0d0[3] = {0, 0, 0}, a0d1[3] = {0, 0, 0};   // A0 demons, planes s, s-1, s-2
+  uint64_t a1d0[3] = {0, 0, 0}, a1d1[3] = {0, 0, 0};   // A1 demons, planes s-1, s-2, s-3
+  uint64_t j1z_prev = ldo(Jz1, (i + yb + LhL * wrapi((int)z0 - 4, L)) * 8u);
+  uint64_t j1x_prev = 0, j1y_prev = 0, j0z_m1 = 0, j0z_m2 = 0;
+
+  const int nstep = (int)a.tz + 6;
+  for (int k = 0; k < nstep; ++k) {
+    const int s = (int)z0 - 3 + k;
+    const uint32_t z = wrapi(s, L), z1 = wrapi(s - 1, L), z2 = wrapi(s - 2, L), z3 = wrapi(s - 3, L);
+    const uint32_t h = i + yb + LhL * z, h1 = i + yb + LhL * z1;
+    const uint32_t h2 = i + yb + LhL * z2, h3 = i + yb + LhL * z3;
+    const uint32_t off = (y + z) & 1u;
+    const uint32_t ip = off ? (i + 1 == Lh ? 0 : i + 1) : i, im = off ? i : (i == 0 ? Lh - 1 : i - 1);
+    const int cur = k & 1, prv = cur ^ 1;
+    // ---- global loads of the step and the LDS writes before the barrier
+    Sp = ldo(S1in, (i + yb + LhL * wrapi(s + 1, L)) * 8u);
+    const uint64_t j0x = ldo(Jx0, h * 8u), j0y = ldo(Jy0, h * 8u), j0z = ldo(Jz0, h * 8u);
+    const uint64_t j1x = ldo(Jx1, h * 8u), j1y = ldo(Jy1, h * 8u), j1z = ldo(Jz1, h * 8u);
+    const uint64_t c0J3 = ldo(Jx1, (im + yb + LhL * z) * 8u), c0J4 = ldo(Jy1, (i + ymb + LhL * z) * 8u);
+    const uint64_t S0v = ldo(S0in, h * 8u);
+    const ulonglong2 dA0 = ldo_stream(Din0, h * 16u);
+    const uint64_t Hv = hrow ? ldo(S1in, (i + hrb + LhL * z) * 8u) : 0;
+    S1p[cur * P + slot] = S0w;
+    XD0[cur * P + row * Lh + iw0] = make_ulonglong2(a0d0[2], a0d1[2]);
+    XD1[cur * P + row * Lh + iw1] = make_ulonglong2(a1d0[2], a1d1[2]);
+    __syncthreads();
+    // ---- A0: first sweep, colour 0, plane s
+    {
+      uint64_t D0 = dA0.x, D1 = dA0.y;
+      const uint64_t *Sl = S1p + cur * P;
+      const uint64_t nxp = Sl[row * Lh + ip], nxm = Sl[row * Lh + im];
+      const uint64_t nyp = row + 1 == ny ? Hv : Sl[rp * Lh + i];
+      const uint64_t nym = row == 0 ? Hv : Sl[rm * Lh + i];
+      E0 = msa::site_update(S0v, nxp, nyp, Sp, nxm, nym, Sm, j0x, j0y, j0z, c0J3, c0J4, j1z_prev,
+                            D0, D1);
+      A0b[cur * P + slot] = E0;
+      a0d0[0] = D0;
+      a0d1[0] = D1;
+    }
+    // ---- A1: first sweep, colour 1, plane s-1
+    if (rowA1 && k >= 2) {
+      const uint64_t c1J3 = ldo(Jx0, (im + yb + LhL * z1) * 8u);
+      const uint64_t c1J4 = ldo(Jy0, (i + ymb + LhL * z1) * 8u);
+      const ulonglong2 d = ldo_stream(Din1, h1 * 16u);
+      uint64_t D0 = d.x, D1 = d.y;
+      const uint64_t *El = A0b + prv * P;
+      F0 = msa::site_update(Sm, El[row * Lh + ip], El[rp * Lh + i], E0, El[row * Lh + im],
+                            El[rm * Lh + i], Em2, j1x_prev, j1y_prev, j1z_prev, c1J3, c1J4, j0z_m2,
+                            D0, D1);
+      A1b[cur * P + slot] = F0;
+      a1d0[0] = D0;
+      a1d1[0] = D1;
+    }
+    // ---- B0: second sweep, colour 0, plane s-2
+    if (rowB0 && k >= 4) {
+      const uint64_t bj0 = ldo(Jx0, h2 * 8u), bj1 = ldo(Jy0, h2 * 8u), bj2 = ldo(Jz0, h2 * 8u);
+      const uint64_t bj3 = ldo(Jx1, (im + yb + LhL * z2) * 8u);
+      const uint64_t bj4 = ldo(Jy1, (i + ymb + LhL * z2) * 8u);
+      const uint64_t bj5 = ldo(Jz1, h3 * 8u);
+      const ulonglong2 d = XD0[cur * P + slot];
+      uint64_t D0 = d.x, D1 = d.y;
+      const uint64_t *Fl = A1b + prv * P;
+      G0 = msa::site_update(Em2, Fl[row * Lh + ip], Fl[rp * Lh + i], F0, Fl[row * Lh + im],
+                            Fl[rm * Lh + i], F2, bj0, bj1, bj2, bj3, bj4, bj5, D0, D1);
+      B0b[cur * P + slot] = G0;
+      if (rowB1 && k >= 5 && k <= (int)a.tz + 4) {
+        sto_stream(S0out, h2 * 8u, G0);
+        sto_stream(Dout0, (iw0 + yb + LhL * z2) * 16u, D0, D1);
+      }
+    }
+    // ---- B1: second sweep, colour 1, plane s-3 (the tile's own rows and planes)
+    if (rowB1 && k >= 6) {
+      const uint32_t z4 = wrapi(s - 4, L);
+      const uint64_t bj0 = ldo(Jx1, h3 * 8u), bj1 = ldo(Jy1, h3 * 8u), bj2 = ldo(Jz1, h3 * 8u);
+      const uint64_t bj3 = ldo(Jx0, (im + yb + LhL * z3) * 8u);
+      const uint64_t bj4 = ldo(Jy0, (i + ymb + LhL * z3) * 8u);
+      const uint64_t bj5 = ldo(Jz0, (i + yb + LhL * z4) * 8u);
+      const ulonglong2 d = XD1[cur * P + slot];
+      uint64_t D0 = d.x, D1 = d.y;
+      const uint64_t *Gl = B0b + prv * P;
+      const uint64_t sn = msa::site_update(F2, Gl[row * Lh + ip], Gl[rp * Lh + i], G0,
+                                           Gl[row * Lh + im], Gl[rm * Lh + i], G2, bj0, bj1, bj2,
+                                           bj3, bj4, bj5, D0, D1);
+      sto_stream(S1out, h3 * 8u, sn);
+      sto_stream(Dout1, (iw1 + yb + LhL * z3) * 16u, D0, D1);
+    }
+    // ---- rotate the windows
+    Sm = S0w;
+    S0w = Sp;
+    Em2 = Em1;
+    Em1 = E0;
+    F2 = F1;
+    F1 = F0;
+    G2 = G1;
+    G1 = G0;
+    a0d0[2] = a0d0[1];
+    a0d1[2] = a0d1[1];
+    a0d0[1] = a0d0[0];
+    a0d1[1] = a0d1[0];
+    a1d0[2] = a1d0[1];
+    a1d1[2] = a1d1[1];
+    a1d0[1] = a1d0[0];
+    a1d1[1] = a1d1[0];
+    j1x_prev = j1x;
+    j1y_prev = j1y;
+    j1z_prev = j1z;
+    j0z_m2 = j0z_m1;
+    j0z_m1 = j0z;
+  }
+}
+
 struct MeasArgs {
   const uint64_t *sp;
   const uint64_t *jp;
@@ -674,6 +867,21 @@ static bool narrow_ok(const Geo &g) {
   return 16ull * g.R * g.nc * g.V < lim && 24ull * g.V < lim && 8ull * g.R * g.V < lim;
 }
 
+// LDS of the sweep-pair kernel: eight planes of 8-byte words and four of
+// 16-byte demons, double-buffered, per thread slot.
+static size_t pair_lds(const Geo &g, uint32_t ty) { return (size_t)128 * (ty + 6) * g.Lh; }
+constexpr size_t kPairLdsMax = 160 * 1024;
+
+// A sweep-pair tile is valid for one demon layer (its demons walk along x,
+// inside a block's rows), 32-bit offsets, ty and tz dividing L and a block
+// (L/2 pairs x (ty + 6) rows) within 1024 threads and the LDS of a CU.
+static int pair_tile_ok(const Geo &g, uint32_t ty, uint32_t tz) {
+  if (g.nc != 1 || !narrow_ok(g) || ty < 1 || tz < 1 || ty > g.L || tz > g.L) return 0;
+  if (g.L % ty || g.L % tz) return 0;
+  if ((uint64_t)g.Lh * (ty + 6) > 1024) return 0;
+  return pair_lds(g, ty) <= kPairLdsMax;
+}
+
 static void set_fused(msa_ctx *c, int mode, uint32_t ty, uint32_t tz) {
   c->fused = mode;
   c->fty = ty;
@@ -740,10 +948,13 @@ extern "C" int msa_create(int device, const msa_geom *geom, msa_ctx **out) {
       hipMemsetAsync(c->jp, 0, 8 * V * 3, c->stream) != hipSuccess ||
       hipStreamSynchronize(c->stream) != hipSuccess)
     return cleanup(msa_fail(MSA_ERR_HIP, "initial clear failed"));
-  // the refresh kernel stages two search tables (37 KiB) in LDS
+  // the refresh kernel stages two search tables (37 KiB) in LDS; the
+  // sweep-pair kernel's buffers take up to a CU's whole LDS
   if (hipFuncSetAttribute(reinterpret_cast<const void *>(&k_halfsweep<kRefresh>),
-                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)kRefreshLds) != hipSuccess)
-    return cleanup(msa_fail(MSA_ERR_HIP, "refresh kernel LDS attribute"));
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)kRefreshLds) != hipSuccess ||
+      hipFuncSetAttribute(reinterpret_cast<const void *>(&k_sweep_pair),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPairLdsMax) != hipSuccess)
+    return cleanup(msa_fail(MSA_ERR_HIP, "kernel LDS attribute"));
   pick_half(c);
   pick_fused(c);
   *out = c;
@@ -771,14 +982,49 @@ extern "C" int msa_destroy(msa_ctx *c) {
   return MSA_OK;
 }
 
+// Default sweep-pair tile: the fewest recomputed sites with at least 256
+// blocks (one replica per block) where possible.
+static int pick_pair(const Geo &g, uint32_t &bty, uint32_t &btz) {
+  double best = -1;
+  bty = btz = 0;
+  for (uint32_t ty = 1; ty <= g.L; ++ty)
+    for (uint32_t tz = 4; tz <= g.L && tz <= 64; ++tz) {
+      if (!pair_tile_ok(g, ty, tz)) continue;
+      const double blocks = (double)(g.L / ty) * (g.L / tz) * g.R;
+      double work = 0;
+      for (int p = 0; p < 4; ++p) work += (ty + 6.0 - 2 * p) * (tz + 6.0 - 2 * p);
+      const double waste = work / (4.0 * ty * tz);
+      const double fill = blocks >= 256 ? 1.0 : blocks / 256.0;
+      const double score = fill / waste;
+      if (score > best) { best = score; bty = ty; btz = tz; }
+    }
+  return bty != 0;
+}
+
 extern "C" int msa_set_tiling(msa_ctx *c, int32_t mode, int32_t ty, int32_t tz) {
   if (!c) return msa_fail(MSA_ERR_ARG, "NULL context");
+  if (mode == 3) {
+    uint32_t y = (uint32_t)ty, z = (uint32_t)tz;
+    if (ty == 0 && tz == 0 && !pick_pair(c->g, y, z))
+      return msa_fail(MSA_ERR_ARG, "no sweep-pair tile for L=%u R=%u nc=%u", c->g.L, c->g.R, c->g.nc);
+    if (!pair_tile_ok(c->g, y, z))
+      return msa_fail(MSA_ERR_ARG, "sweep-pair tile ty=%d tz=%d invalid for L=%u R=%u nc=%u", ty, tz,
+                      c->g.L, c->g.R, c->g.nc);
+    // a sweep left over between refreshes runs on the one-pass kernel where
+    // it exists for this R, else on the two-pass form
+    if (c->fused == 0) pick_fused(c);
+    c->pair = 1;
+    c->pty = y;
+    c->ptz = z;
+    return MSA_OK;
+  }
+  c->pair = 0;
   if (mode == 0) {
     c->fused = 0;
     return MSA_OK;
   }
   if (mode != 1 && mode != 2)
-    return msa_fail(MSA_ERR_ARG, "mode=%d (0 two-pass, 1 and 2 one-pass)", mode);
+    return msa_fail(MSA_ERR_ARG, "mode=%d (0 two-pass, 1 and 2 one-pass, 3 sweep pairs)", mode);
   if (mode == 2 && !narrow_ok(c->g))
     return msa_fail(MSA_ERR_ARG, "mode 2 needs every buffer below 4 GiB (L=%u R=%u nc=%u)", c->g.L,
                     c->g.R, c->g.nc);
@@ -796,9 +1042,9 @@ extern "C" int msa_set_tiling(msa_ctx *c, int32_t mode, int32_t ty, int32_t tz) 
 
 extern "C" int msa_get_tiling(msa_ctx *c, int32_t *mode, int32_t *ty, int32_t *tz) {
   if (!c || !mode || !ty || !tz) return msa_fail(MSA_ERR_ARG, "NULL argument");
-  *mode = c->fused;
-  *ty = (int32_t)c->fty;
-  *tz = (int32_t)c->ftz;
+  *mode = c->pair ? 3 : c->fused;
+  *ty = (int32_t)(c->pair ? c->pty : c->fty);
+  *tz = (int32_t)(c->pair ? c->ptz : c->ftz);
   return MSA_OK;
 }
 
@@ -982,6 +1228,26 @@ static void launch_fused(msa_ctx *c, const FusedArgs &fa) {
   }
 }
 
+// Two refresh-free iterations of a one-layer lattice in one pass.  The pass
+// reads the current buffers and writes the other ones, so each ping-pong
+// index flips once for the two iterations.
+static void launch_pair(msa_ctx *c) {
+  PairArgs pa{};
+  pa.sp_in = c->sp[c->scur];
+  pa.sp_out = c->sp[c->scur ^ 1];
+  pa.jp = c->jp;
+  pa.dm_in = c->dm[c->cur];
+  pa.dm_out = c->dm[c->cur ^ 1];
+  pa.g = c->g;
+  pa.ty = c->pty;
+  pa.tz = c->ptz;
+  pa.ny = c->pty + 6;
+  const dim3 grid((c->g.L / c->pty) * (c->g.L / c->ptz) * c->g.R), block(c->g.Lh * pa.ny);
+  hipLaunchKernelGGL(k_sweep_pair, grid, block, pair_lds(c->g, c->pty), c->stream, pa);
+  c->scur ^= 1;
+  c->cur ^= 1;
+}
+
 static int launch_run(msa_ctx *c, uint64_t t0, int64_t niter, int64_t every) {
   if (niter < 0) return msa_fail(MSA_ERR_ARG, "niter=%lld", (long long)niter);
   if (every < -1) return msa_fail(MSA_ERR_ARG, "refresh_every=%lld", (long long)every);
@@ -991,6 +1257,11 @@ static int launch_run(msa_ctx *c, uint64_t t0, int64_t niter, int64_t every) {
     if (k >= 0 && (k >= c->nref || c->nlut == 0))
       return msa_fail(MSA_ERR_RANGE, "iteration %llu needs refresh %lld but %d are set",
                   (unsigned long long)t, (long long)k, c->nref);
+    if (k < 0 && c->pair && n + 1 < niter && refresh_index(t + 1, every) < 0) {
+      launch_pair(c);
+      ++n;
+      continue;
+    }
     const ulonglong2 *dmi = c->dm[c->cur];
     ulonglong2 *dmo = c->dm[c->cur ^ 1];
     for (uint32_t l = 0; l < c->g.nc; ++l) {
@@ -1237,6 +1508,9 @@ extern "C" int msa_autotune(msa_ctx *c, int32_t sweeps, int32_t *mode, int32_t *
       if (fused_tile_ok(c->g, y, z))
         for (int32_t m = 1; m <= (narrow_ok(c->g) ? 2 : 1); ++m)
           cands.push_back({m, (int32_t)y, (int32_t)z});
+  for (uint32_t y = 1; y <= c->g.L; ++y)
+    for (uint32_t z = 4; z <= c->g.L && z <= 64; ++z)
+      if (pair_tile_ok(c->g, y, z)) cands.push_back({3, (int32_t)y, (int32_t)z});
   msa_ctx *t = nullptr;
   int rc = msa_create(c->device, &c->geom, &t);
   if (rc) return rc;

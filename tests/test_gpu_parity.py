@@ -196,3 +196,62 @@ def test_one_pass_kernels_agree_at_larger_sizes(pkg, L, R, ty, tz):
             assert np.array_equal(x, y)
     for d in devs:
         d.close()
+
+
+@pytest.mark.parametrize("L,R,ty,tz", [
+    (16, 1, 16, 16),     # whole planes: rows and planes alias onto the block itself
+    (16, 2, 4, 4),
+    (8, 1, 2, 2),        # the pipeline (tz + 6 planes) wraps the whole lattice
+    (24, 2, 3, 8),
+    (32, 4, 8, 16),
+    (40, 3, 5, 10),      # odd R, XCD placement of (tile, replica) pairs
+])
+def test_sweep_pairs_match_oracle(pkg, orc, L, R, ty, tz):
+    """Mode 3: refresh-free iterations run two per pass; runs of 1..4
+    iterations with refreshes every 4 give pairs, single sweeps and refresh
+    sweeps in every order."""
+    print("case L=%d R=%d pair tile %dx%d" % (L, R, ty, tz), flush=True)
+    o = random_state(L, R, 1, 777 + L + R)
+    every = 4
+    chunks = [1, 2, 3, 4, 2, 1, 4, 3]
+    niter = sum(chunks)
+    times = list(range(0, niter, every))
+    ctr, keys, raw, luts = oracle_schedule(orc, R, times, [0.9] * len(times), jobid=2)
+    dev = pkg.Lattice(L, R, 1)
+    dev.set_tiling(3, ty, tz)
+    assert dev.tiling() == (3, ty, tz)
+    dev.upload(o.J, o.S, o.D0, o.D1)
+    dev.set_luts(raw)
+    dev.set_refresh(ctr, keys, np.arange(len(times), dtype=np.int32))
+    t = 0
+    for n in chunks:
+        dev.run(t, n, every)
+        for u in range(t, t + n):
+            if u % every == 0:
+                orc.iteration(o, ctr[u // every], keys, luts[u // every])
+            else:
+                orc.iteration(o)
+        t += n
+        assert_same_state(dev, o, "after t=%d" % t)
+    assert_same_measures(dev, orc, o)
+    dev.close()
+
+
+@pytest.mark.parametrize("L,R,ty,tz", [(96, 2, 12, 8), (64, 1, 16, 16), (160, 2, 5, 20)])
+def test_sweep_pairs_agree_at_larger_sizes(pkg, L, R, ty, tz):
+    """Mode 3 against the two-pass form on bigger lattices, 7 refresh-free
+    sweeps (three pairs and one single sweep) from a Philox state."""
+    print("case L=%d R=%d pair tile %dx%d" % (L, R, ty, tz), flush=True)
+    devs = [pkg.Lattice(L, R, 1) for _ in range(2)]
+    rng = np.random.default_rng(L + 1)
+    d0, d1 = (rng.integers(0, 2 ** 64, size=(R, 1, L ** 3), dtype=np.uint64) for _ in range(2))
+    devs[0].set_tiling(0)
+    devs[1].set_tiling(3, ty, tz)
+    for d in devs:
+        d.init_philox(4321)
+        d.upload(d.couplings(), d.download()[0], d0, d1)
+        d.run(0, 7, 0)
+    for x, y in zip(devs[1].download(), devs[0].download()):
+        assert np.array_equal(x, y)
+    for d in devs:
+        d.close()
