@@ -136,6 +136,31 @@ int msa_run_thermal(msa_ctx *ctx, int64_t niter, const msa_lut *lut, const msa_u
  * q and qlink have no reference counterpart.  Blocks until done. */
 int msa_measure(msa_ctx *ctx, int32_t *unsat, int32_t *demon, int32_t *q, int32_t *qlink);
 
+/* Demon refresh from the reference's walker streams (InitWalkers,
+ * ini.c:712-781, run on host threads there, refresh_walkers main.cu:340-377):
+ * per (sample, replica) one sequential stream of the PR+LCG and xoshiro256++
+ * generators, 64 draws per multispin word, MSC_V * nc words per refresh, demon
+ * = number of thresholds the sum of the two outputs lies below.  The state of
+ * a stream: ring = the last 61 values of the PR lagged sum (x_{n-61} ..
+ * x_{n-1}), lcg = the LCG state, xo = the xoshiro256++ state. */
+typedef struct {
+  uint64_t ring[61];
+  uint64_t lcg;
+  uint64_t xo[4];
+} msa_walker_stream;
+
+/* Replace the Philox refresh of msa_run by the walker streams: n streams per
+ * replica, lanes[i] the lane of stream i, st[i*R + r] their states, thr[k][3]
+ * the thresholds of refresh k (get_thresholds, ini.c:680-710).  Refresh k
+ * writes every demon (all layers, both colours) of the listed lanes from the
+ * next segment of each stream and zeros in the other lanes; refresh indices
+ * follow msa_run's cadence.  L must be a multiple of 8.  n = 0 returns to
+ * the Philox refresh.  Bit-identical to the reference's walkers. */
+int msa_set_walker_streams(msa_ctx *ctx, int32_t n, const int32_t *lanes,
+                           const msa_walker_stream *st, int32_t nref, const uint64_t *thr);
+/* The streams' current states (after the refreshes run so far), st[n*R]. */
+int msa_get_walker_streams(msa_ctx *ctx, msa_walker_stream *st);
+
 /* Kernel choice for sweeps without a refresh (no reference counterpart; the
  * autotuner and the batched form set it).  mode 0: two kernels per sweep,
  * one per colour.  mode 1: one kernel per sweep whose blocks own ty rows x

@@ -161,6 +161,21 @@ int  msa_read_schedule(const char *path, msa_sched *s, char *err, int errlen);
  * mean of x_r, sqrt((<x^2> - <x>^2) / (n - 1)). */
 void msa_mean_err(const double *x, int n, double *mean, double *err);
 
+/* ----------------------------------------------------- walker streams --
+ * Host side of the reference's demon streams (InitWalkers, ini.c:712-781),
+ * run on the device by msa_set_walker_streams (include/msa.h). */
+
+/* Init_Random (ini.c:53-58): stream (i, r) is Inicia_generadores_CPU of
+ * seeds_C[i*R + r], seeds_C = generate_seeds_from_one(seed_u, ...);
+ * pr[i*R + r], xo[i*R + r] for i < n. */
+void msa_walker_generators(uint64_t seed_u, int n, int R, msa_prng *pr, msa_xo256 *xo);
+/* A generator pair as the device's stream state, and back: p is the ring
+ * pointer of the rebuilt PR+LCG generator (a refresh of V*nc draws, V a
+ * multiple of 256, leaves the pointer where it was); the rebuilt generators
+ * continue the same streams and carry the last outputs as `out`. */
+void msa_walker_stream_of(const msa_prng *pr, const msa_xo256 *xo, msa_walker_stream *st);
+void msa_walker_stream_to(const msa_walker_stream *st, uint8_t p, msa_prng *pr, msa_xo256 *xo);
+
 /* -------------------------------------------------------- checkpoints --
  * The reference's checkpoint files (io.c:17-393) with NBETAS = 1 and NR = R
  * replicas, byte for byte; L must be a multiple of 8 (its multispin layout).

@@ -32,6 +32,15 @@ struct msa_ctx {
   int fyhalo = 1;
   int pair = 0;                           // refresh-free sweeps two per pass (k_sweep_pair)
   uint32_t pty = 0, ptz = 0;              // its tile: rows, planes
+  // demon refresh from the reference's walker streams (msa_walkers.hip)
+  int32_t w_n = 0;                        // streams per replica, 0 = Philox refresh
+  int w_cur = 0;                          // w_st buffer holding the current states
+  msa_walker_stream *w_st[2] = {nullptr, nullptr};
+  void *w_cs = nullptr;                   // per-chunk start states
+  int32_t *w_slot = nullptr;              // [64] stream of each lane
+  uint64_t *w_lf = nullptr, *w_lcg = nullptr, *w_xo = nullptr;   // jump tables
+  uint32_t w_W = 0, w_nchunk = 0;         // words per chunk, chunks per segment
+  std::vector<uint64_t> w_thr;            // [nref][3] thresholds
 };
 
 
@@ -41,6 +50,10 @@ int msa_ctx_share_luts(msa_ctx *c, msa::FineLut *luts, int32_t nlut);
 // Upload nlut LUT pairs and build their search tables (2 * nlut msa::FineLut)
 // in new device memory *out; synchronous on `st`.
 int msa_build_fine_luts(hipStream_t st, int32_t nlut, const msa_lut *host, msa::FineLut **out);
+
+// Walker-stream refresh k of the current demon buffer (msa_walkers.hip).
+int walkers_refresh(msa_ctx *c, int64_t k);
+void walkers_free(msa_ctx *c);
 
 // Dynamic LDS of the refresh kernels (two search tables).
 constexpr size_t kRefreshLds = 2 * sizeof(msa::FineLut);

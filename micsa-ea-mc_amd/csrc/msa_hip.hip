@@ -965,6 +965,7 @@ extern "C" int msa_destroy(msa_ctx *c) {
   if (!c) return MSA_OK;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  walkers_free(c);
   (void)hipFree(c->sp[0]);
   (void)hipFree(c->sp[1]);
   (void)hipFree(c->jp);
@@ -1253,7 +1254,12 @@ static int launch_run(msa_ctx *c, uint64_t t0, int64_t niter, int64_t every) {
   if (every < -1) return msa_fail(MSA_ERR_ARG, "refresh_every=%lld", (long long)every);
   for (int64_t n = 0; n < niter; ++n) {
     const uint64_t t = t0 + (uint64_t)n;
-    const int64_t k = refresh_index(t, every);
+    int64_t k = refresh_index(t, every);
+    if (k >= 0 && c->w_n > 0) {           // the reference's walker streams
+      const int rc = walkers_refresh(c, k);
+      if (rc) return rc;
+      k = -1;
+    }
     if (k >= 0 && (k >= c->nref || c->nlut == 0))
       return msa_fail(MSA_ERR_RANGE, "iteration %llu needs refresh %lld but %d are set",
                   (unsigned long long)t, (long long)k, c->nref);

@@ -71,6 +71,8 @@ def hip():
         L.msa_set_tiling.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32]
         L.msa_get_tiling.argtypes = [vp, i32p, i32p, i32p]
         L.msa_run_thermal.argtypes = [vp, C.c_int64, C.c_void_p, u32p, u32p]
+        L.msa_set_walker_streams.argtypes = [vp, C.c_int32, i32p, C.c_void_p, C.c_int32, u64p]
+        L.msa_get_walker_streams.argtypes = [vp, C.c_void_p]
         bp = C.c_void_p
         L.msa_batch_create.argtypes = [C.c_int, C.c_int64, C.POINTER(bp)]
         L.msa_batch_destroy.argtypes = [bp]
@@ -170,6 +172,22 @@ class Lattice:
     def run(self, t0, niter, refresh_every):
         _check(self.lib.msa_run(self.ctx, t0, niter, refresh_every))
 
+    def set_walker_streams(self, lanes, streams, thr):
+        """Demon refresh from the reference's walker streams (msa_set_walker_streams):
+        lanes (n,), streams: n*R WalkerStream (stream i, replica r at i*R + r),
+        thr (nref, 3) uint64 thresholds.  Empty lanes return to the Philox refresh."""
+        lanes = np.ascontiguousarray(lanes, np.int32)
+        thr = np.ascontiguousarray(thr, np.uint64).reshape(-1, 3)
+        arr = (WalkerStream * max(1, len(streams)))(*streams)
+        _check(self.lib.msa_set_walker_streams(self.ctx, lanes.size, _p(lanes, i32p),
+                                               C.cast(arr, C.c_void_p), thr.shape[0], _p(thr)))
+        self._walker_n = lanes.size
+
+    def walker_streams(self):
+        arr = (WalkerStream * (self._walker_n * self.R))()
+        _check(self.lib.msa_get_walker_streams(self.ctx, C.cast(arr, C.c_void_p)))
+        return list(arr)
+
     def run_thermal(self, lut, ctr, key):
         """Canonical Metropolis thermalization (msa_run_thermal): `lut` is the
         4096 bytes of the heat-bath table, ctr (nterm, 2, R, 4), key (R,)."""
@@ -240,6 +258,11 @@ class Xo256(C.Structure):
     _fields_ = [("s", C.c_uint64 * 4), ("out", C.c_uint64)]
 
 
+class WalkerStream(C.Structure):
+    """msa_walker_stream: last 61 lagged-sum values, LCG state, xoshiro256++ state."""
+    _fields_ = [("ring", C.c_uint64 * 61), ("lcg", C.c_uint64), ("xo", C.c_uint64 * 4)]
+
+
 class SData(C.Structure):
     """s_data as stored in conf files (184 bytes)."""
     _fields_ = [("maxtime", C.c_uint64), ("now", C.c_uint64), ("ntimes", C.c_int32),
@@ -302,6 +325,11 @@ def host():
                                               C.POINTER(Prng), C.POINTER(Xo256), C.c_uint32, u32p,
                                               C.c_void_p, i32p]
         L.msa_gen_skip.argtypes = [C.POINTER(Prng), C.POINTER(Xo256), C.c_int, C.c_int64]
+        L.msa_walker_generators.argtypes = [C.c_uint64, C.c_int, C.c_int, C.POINTER(Prng),
+                                            C.POINTER(Xo256)]
+        L.msa_walker_stream_of.argtypes = [C.POINTER(Prng), C.POINTER(Xo256), C.POINTER(WalkerStream)]
+        L.msa_walker_stream_to.argtypes = [C.POINTER(WalkerStream), C.c_uint8, C.POINTER(Prng),
+                                           C.POINTER(Xo256)]
         u8p = C.POINTER(C.c_uint8)
         L.msa_conf_bytes.argtypes = [C.c_int]
         L.msa_conf_bytes.restype = C.c_uint64
@@ -631,3 +659,24 @@ def walkers_read(path, L, R, nc):
     if host().msa_walkers_read(path.encode(), L, R, nc, C.byref(ic), pa, xa, _p(w), e, len(e)):
         raise MsaError(e.value.decode())
     return ic, list(pa), list(xa), w
+
+
+# ---------------------------------------------------------- walker streams --
+def walker_streams(seed_u, n, R):
+    """The reference's walker streams of n samples x R replicas (stream i*R + r):
+    (list of WalkerStream, list of (Prng, Xo256) they started from)."""
+    H = host()
+    pr, xo = (Prng * (n * R))(), (Xo256 * (n * R))()
+    H.msa_walker_generators(seed_u, n, R, pr, xo)
+    out = []
+    for k in range(n * R):
+        st = WalkerStream()
+        H.msa_walker_stream_of(C.byref(pr[k]), C.byref(xo[k]), C.byref(st))
+        out.append(st)
+    return out, [(pr[k], xo[k]) for k in range(n * R)]
+
+
+def thresholds(beta):
+    t = np.zeros(3, np.uint64)
+    host().msa_thresholds(beta, _p(t))
+    return t

@@ -159,6 +159,10 @@ void     orc_msc_pack(orc_msc *m, const uint8_t *spin, const uint8_t *jx, const 
                       const uint8_t *jz, const uint8_t *d0, const uint8_t *d1);
 void     orc_msc_unpack(const orc_msc *m, uint8_t *spin, uint8_t *d0, uint8_t *d1);
 void     orc_msc_walker_words(const orc_msc *m, uint64_t *out /*[4*nc*HALF*R]*/);
+/* InitWalkers (ini.c:712-781) for one replica: nwords = MSC_V * nc words,
+ * w0/w1 in iword order; advances pr and xo by 64 * nwords draws. */
+void     orc_init_walkers(orc_prlcg *pr, orc_xoshiro *xo, const uint64_t thr[3], uint64_t nwords,
+                          uint64_t *w0, uint64_t *w1);
 void     orc_msc_monte_carlo(orc_msc *m);       /* main.cu:254-306 (micro branch) */
 void     orc_msc_thermal(orc_msc *m, const orc_lut *lut, const uint32_t *ctr0,
                          const uint32_t *ctr1, const uint32_t *key);   /* main.cu:202-252 */

@@ -101,6 +101,8 @@ class Oracle:
         L.orc_prlcg_init_hq.argtypes = [C.POINTER(PrLcg), C.c_uint64]
         L.orc_init_cpu_generators.argtypes = [C.POINTER(PrLcg), C.POINTER(Xoshiro), u32p, C.c_uint64]
         L.orc_seeds_from_one.argtypes = [C.c_uint64, u64p, C.c_int]
+        L.orc_init_walkers.argtypes = [C.POINTER(PrLcg), C.POINTER(Xoshiro), u64p, C.c_uint64,
+                                       u64p, u64p]
         L.orc_janus_init.argtypes = [C.POINTER(Janus), C.c_uint64]
         L.orc_janus_next.argtypes = [C.POINTER(Janus)]
         L.orc_janus_next.restype = C.c_uint64
@@ -211,6 +213,14 @@ class Oracle:
         k = np.array(key, np.uint32)
         w = self.lib.orc_rnd_word(C.byref(lut), _p(c, u32p), _p(k, u32p))
         return w, [int(x) for x in c]
+
+    def init_walkers(self, pr, xo, thr, nwords):
+        """InitWalkers for one replica: (w0, w1) uint64 [nwords]; advances pr, xo."""
+        w0 = np.zeros(nwords, np.uint64)
+        w1 = np.zeros(nwords, np.uint64)
+        t = np.ascontiguousarray(thr, np.uint64)
+        self.lib.orc_init_walkers(C.byref(pr), C.byref(xo), _p(t), nwords, _p(w0), _p(w1))
+        return w0, w1
 
     def fill_counter(self, pr, xo, it, parity, jobid):
         c = np.zeros(4, np.uint32)
