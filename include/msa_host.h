@@ -78,6 +78,12 @@ void msa_build_couplings(int L, uint64_t seed_J, uint64_t *J);
 /* Spins: Init_u (ini.c:275-299) with the 64 lanes as the sample slots:
  * one Init_Rand_HQ_64bits(seed_u) stream, replica, site, lane order. */
 void msa_build_spins(int L, int R, uint64_t seed_u, uint64_t *spin);
+/* The reference's default (JANUS) build: all samples share the J of one
+ * seed (io.c:797-799 overrides seeds_J), lanes identical. */
+void msa_build_couplings_one(int L, uint64_t seed, uint64_t *J);
+/* Init_u with nslot sample slots per site (MAXNSAMPLES = 2 in the reference):
+ * lane b < nslot holds slot b's sample, other lanes zero. */
+void msa_build_spins_slots(int L, int R, uint64_t seed_u, int nslot, uint64_t *spin);
 
 /* ------------------------------------------------------------ schedule -- */
 
@@ -154,7 +160,8 @@ int  msa_read_betas(const char *path, double *beta, char *err, int errlen);   /*
 int  msa_read_lut_file(const char *path, double beta, msa_lut *lut, char *err, int errlen);
 int  msa_read_times(const char *path, int n, unsigned long long *times, char *err, int errlen);
 int  msa_read_samples(const char *path, int n, int *samples, char *err, int errlen);
-/* annealing schedule file: T0 T1 nsweeps refresh_every (one per line) */
+/* annealing schedule file: T0 T1 nsweeps refresh_every (one per line);
+ * T0 = T1 = 0 keeps the inverse temperature of beta.dat */
 int  msa_read_schedule(const char *path, msa_sched *s, char *err, int errlen);
 
 /* Mean and error over replicas as Measures does (med.c:13-51):

@@ -19,9 +19,18 @@
  *   device       GPU index
  *   max_time     wall-clock budget in seconds, 0 = none (timing.c:35-58)
  *   list_samples lanes to write (io.c:593-624)
- *   schedule.in  linear anneal "T0 T1 nsweeps refresh_every"; without it the
- *                demons are refreshed at fixed beta (beta.dat) at iterations
- *                0, 1, 2, 4, 8, ... as the reference does (main.cu:140-142)
+ *   schedule.in  linear anneal "T0 T1 nsweeps refresh_every" (T0 = T1 = 0:
+ *                fixed beta from beta.dat), with the Philox demon refresh on
+ *                all 64 lanes
+ *
+ * Without schedule.in the run is the reference program's own (default JANUS
+ * build, NBETAS = 1, MAXNSAMPLES = 2) bit for bit: all samples share the J of
+ * the JANUS seed (io.c:797-799), spins from Init_u with two sample slots
+ * (ini.c:275-299), demons refreshed at iterations 0, 1, 2, 4, ...
+ * (main.cu:140-142) from the reference's walker streams (InitWalkers,
+ * ini.c:712-781, on the device), a restart repeats the thermalization as the
+ * reference does (main.cu:124-128).  Its limits apply: nbits <= 2, samples
+ * 0 and 1, L a multiple of 8.
  *
  * Files under <dir>I%03d/ as the reference writes them: master_seedJ.in,
  * run.log, the `running` lock, `error`; BIT%03d/seeds.in, evol_*.txt and
@@ -29,12 +38,11 @@
  * Checkpoints are written at every measurement time when L is a multiple
  * of 8 (the reference's multispin layout, which the walkers file uses).
  *
- * Differences from the reference, by design: demons are refreshed on the GPU
- * from Philox4x32-10 + LUT bits (no host threads), so the generators stored
- * in conf and walkers files are the per-replica generators those refreshes
- * draw their counters from, at the checkpoint's iteration.  A restart does
- * not repeat the thermalization (the reference does, main.cu:124-128), so a
- * restarted run continues bit for bit as if it had not stopped.  Signals
+ * With schedule.in, by design: demons are refreshed on the GPU from
+ * Philox4x32-10 + LUT bits, so the generators stored in conf and walkers
+ * files are the per-replica generators those refreshes draw their counters
+ * from, at the checkpoint's iteration, and a restart does not repeat the
+ * thermalization, so it continues bit for bit as if it had not stopped.  Signals
  * (SIGTERM, SIGINT, SIGUSR1) stop the run at the next chunk boundary, after
  * the device has finished its queued work, with the reference's error file
  * and lock removal (io.c:970-1014).  The canonical Metropolis thermalization
@@ -236,8 +244,13 @@ typedef struct {
   char (*repdir)[MSA_MAXR][PATHLEN + 8];
   const msa_input *in;
   double beta;
+  int refmode;                  /* the reference's own workload (no schedule.in) */
+  uint8_t walker_p;             /* ring pointer of the walker generators */
   uint64_t seeds_J[64], seeds_C[64];
 } run_files;
+
+/* io.c:797-799 (JANUS build): every sample's J seed */
+#define JANUS_SEED_J 9164166177945067407ULL
 
 /* check_and_prepare_simulation_backup, io.c:690-752: counts the conf files of
  * the listed samples and resolves flag 2.  Returns the resolved flag and the
@@ -300,6 +313,8 @@ static int get_seeds(run_files *rf, int flag, uint64_t *seed_J, uint64_t *seed_u
   }
   *seed_J = nonzero_seed(*seed_J);
   msa_seeds_from_one(*seed_J, rf->seeds_J, 64);
+  if (rf->refmode)
+    for (int b = 0; b < 64; b++) rf->seeds_J[b] = JANUS_SEED_J;
   for (int i = 0; i < rf->nbits; i++) {
     countJ0[i] = 0;
     snprintf(name, sizeof name, "%sseeds.in", rf->bitdir[i]);
@@ -384,7 +399,7 @@ static void fill_icds(const run_files *rf, int i, msa_icds *ic)
  * conf files are returned; every stored energy is checked against the spins. */
 static unsigned long long read_checkpoint(const run_files *rf, const uint64_t *J, uint64_t *S,
                                           uint64_t *d0, uint64_t *d1, msa_prng *pr,
-                                          msa_xo256 *xo)
+                                          msa_xo256 *xo, msa_prng *wpr, msa_xo256 *wxo)
 {
   const int L = rf->L, R = rf->R, nc = rf->nc, n = rf->nbits;
   const uint64_t V = (uint64_t)L * L * L, V8 = V / 8, nw = msa_msc_words(L, R, nc);
@@ -444,9 +459,14 @@ static unsigned long long read_checkpoint(const run_files *rf, const uint64_t *J
         ic.walkers_per_site != want.walkers_per_site || ic.lv != want.lv ||
         ic.seed != want.seed || fabs(ic.beta - want.beta) > 1e-6)
       die("Different icds from simulation and %s", name);
-    for (int r = 0; r < R; r++)
-      if (!gens_equal(&wp[r], &wx[r], &pr[r], &xo[r]))
+    for (int r = 0; r < R; r++) {
+      if (rf->refmode) {                /* the walker streams of this sample */
+        wpr[i * R + r] = wp[r];
+        wxo[i * R + r] = wx[r];
+      } else if (!gens_equal(&wp[r], &wx[r], &pr[r], &xo[r])) {
         die("Generators of replica %d in %s differ from those of the conf files", r, name);
+      }
+    }
   }
   msa_msc_to_lanes(L, R, nc, w, n, rf->samples, d0, d1);
   free(w);
@@ -461,7 +481,7 @@ static void write_checkpoint(const run_files *rf, unsigned long long now,
                              unsigned long long prev, int flag, uint64_t seed_u,
                              const uint64_t *seed_MC, const uint64_t *S, const uint64_t *d0,
                              const uint64_t *d1, const int32_t *energy, const msa_prng *pr,
-                             const msa_xo256 *xo)
+                             const msa_xo256 *xo, const msa_prng *wpr, const msa_xo256 *wxo)
 {
   const int L = rf->L, R = rf->R, nc = rf->nc, n = rf->nbits;
   const uint64_t V8 = (uint64_t)L * L * L / 8, nw = msa_msc_words(L, R, nc);
@@ -471,8 +491,8 @@ static void write_checkpoint(const run_files *rf, unsigned long long now,
   for (int i = 0; i < n; i++) {
     msa_icds ic;
     fill_icds(rf, i, &ic);
-    if (msa_walkers_write(rf->bitdir[i], now, prev, &ic, L, R, nc, pr, xo,
-                          w + (uint64_t)i * nw, e, sizeof e))
+    if (msa_walkers_write(rf->bitdir[i], now, prev, &ic, L, R, nc, wpr ? wpr + i * R : pr,
+                          wxo ? wxo + i * R : xo, w + (uint64_t)i * nw, e, sizeof e))
       die("%s\nProblems writing walkers", e);
   }
   free(w);
@@ -529,6 +549,15 @@ int main(int argc, char **argv)
   if (in.nc < 1 || in.nc > 6) die("nc=%d outside 1..6", in.nc);
   const int L = in.l & 0xffff, R = in.nr;
   const int can_ckpt = (L % 8 == 0);
+  const int refmode = f_sched == NULL;
+  if (refmode) {
+    /* the reference's limits (header.h:20, main.cu:64-65, ini.c:281-295) */
+    if (nbits > 2) die("Error: nbits out of range, 0 < bit < %d", 2);
+    for (int i = 0; i < nbits; i++)
+      if (samples[i] > 1 || (i > 0 && samples[i] <= samples[i - 1]))
+        die("Problems generating spins (samples must be 0 and/or 1, increasing)");
+    if (L % 8) die("L must be a multiple of 8");
+  }
 
   printf("Path: %s\n", in.dir);
   msa_print_input(stdout, &in);
@@ -538,6 +567,7 @@ int main(int argc, char **argv)
   memset(&sch, 0, sizeof sch);
   if (f_sched) {
     if (msa_read_schedule(f_sched, &sch, e, sizeof e)) die("%s", e);
+    if (sch.T0 <= 0) sch.beta_fixed = beta;
   } else {
     sch.refresh_every = -1;
     sch.beta_fixed = beta;
@@ -570,6 +600,7 @@ int main(int argc, char **argv)
   rf.repdir = repdir;
   rf.in = &in;
   rf.beta = beta;
+  rf.refmode = refmode;
 
   unsigned long long conf_now = 0;
   const int flag = can_ckpt ? prepare_backup(&rf, in.flag, can_ckpt, &conf_now)
@@ -597,8 +628,13 @@ int main(int argc, char **argv)
   uint64_t *D0 = (uint64_t *)calloc((size_t)((uint64_t)R * in.nc * V), sizeof(uint64_t));
   uint64_t *D1 = (uint64_t *)calloc((size_t)((uint64_t)R * in.nc * V), sizeof(uint64_t));
   if (!D0 || !D1) die("Problems allocating host demons");
-  msa_build_couplings(L, seed_J, J);
-  msa_build_spins(L, R, seed_u, S);
+  if (refmode) {
+    msa_build_couplings_one(L, JANUS_SEED_J, J);
+    msa_build_spins_slots(L, R, seed_u, 2, S);
+  } else {
+    msa_build_couplings(L, seed_J, J);
+    msa_build_spins(L, R, seed_u, S);
+  }
 
   /* countJ check of Init_J, ini.c:500-511 */
   int32_t countJ[64];
@@ -616,9 +652,20 @@ int main(int argc, char **argv)
   uint32_t keys[MSA_MAXR];
   for (int r = 0; r < R; r++) msa_cpu_generators(&pr[r], &xo[r], &keys[r], seed_MC[r]);
 
+  /* the walker streams of the reference's demon refresh (Init_Random,
+   * ini.c:53-58), per listed sample and replica */
+  msa_prng *wpr = NULL;
+  msa_xo256 *wxo = NULL;
+  if (refmode) {
+    wpr = (msa_prng *)xmalloc(sizeof(msa_prng) * nbits * R, "walker generators");
+    wxo = (msa_xo256 *)xmalloc(sizeof(msa_xo256) * nbits * R, "walker generators");
+    msa_walker_generators(seed_u, nbits, R, wpr, wxo);
+    rf.walker_p = wpr[0].p;
+  }
+
   unsigned long long start = in.now;
   if (flag == 0) {
-    start = read_checkpoint(&rf, J, S, D0, D1, pr, xo);
+    start = read_checkpoint(&rf, J, S, D0, D1, pr, xo, wpr, wxo);
     msa_print_input(stdout, &in);
     fflush(stdout);
     if (start >= in.maxtime) die("There are %llu iterations in %s\nProblems in backup", start, g_dir);
@@ -630,8 +677,9 @@ int main(int argc, char **argv)
   CHK(msa_create(device, &g, &ctx));
   CHK(msa_upload_state(ctx, J, S, flag == 0 ? D0 : NULL, flag == 0 ? D1 : NULL));
 
-  if (in.nterm > 0 && flag == 1) {
-    /* THERMALIZATION, main.cu:124-128: Monte_Carlo(flag = 0) nterm times */
+  if (in.nterm > 0 && (flag == 1 || refmode)) {
+    /* THERMALIZATION, main.cu:124-128: Monte_Carlo(flag = 0) nterm times, on a
+     * restart too in the reference's own workload */
     printf("Thermalization: %d Metropolis iterations at beta = %.6f\n", in.nterm, beta);
     fflush(stdout);
     msa_u32x4 *tctr = (msa_u32x4 *)xmalloc(sizeof(msa_u32x4) * 2 * (size_t)in.nterm * R,
@@ -653,15 +701,29 @@ int main(int argc, char **argv)
 
   const int64_t niter = (int64_t)in.maxtime;
   const int64_t nref = msa_sched_count(&sch, niter > 0 ? niter : 1);
-  msa_u32x4 *ctr = (msa_u32x4 *)xmalloc(sizeof(msa_u32x4) * (size_t)(nref * R), "refresh counters");
-  msa_lut *luts = (msa_lut *)xmalloc(sizeof(msa_lut) * 2 * (size_t)nref, "refresh tables");
-  int32_t *lut_idx = (int32_t *)xmalloc(sizeof(int32_t) * (size_t)nref, "refresh indices");
-  msa_sched_tables_resume(&sch, k_start, nref, R, pr, xo, (uint32_t)sample, ctr, luts, lut_idx);
-  CHK(msa_set_luts(ctx, (int32_t)nref, luts));
-  CHK(msa_set_refresh(ctx, (int32_t)nref, ctr, keys, lut_idx));
-  free(luts);
-  free(ctr);
-  free(lut_idx);
+  if (refmode) {
+    /* refresh_walkers (main.cu:340-377): refresh k takes the next segment of
+     * each walker stream, with get_thresholds at beta.dat (ini.c:680-710) */
+    msa_walker_stream *st = (msa_walker_stream *)xmalloc(sizeof(msa_walker_stream) * nbits * R,
+                                                          "walker streams");
+    uint64_t *thr = (uint64_t *)xmalloc(sizeof(uint64_t) * 3 * (size_t)nref, "thresholds");
+    for (int k = 0; k < nbits * R; k++) msa_walker_stream_of(&wpr[k], &wxo[k], &st[k]);
+    msa_thresholds(beta, thr);
+    for (int64_t k = 1; k < nref; k++) memcpy(thr + 3 * k, thr, 3 * sizeof(uint64_t));
+    CHK(msa_set_walker_streams(ctx, nbits, samples, st, (int32_t)nref, thr));
+    free(st);
+    free(thr);
+  } else {
+    msa_u32x4 *ctr = (msa_u32x4 *)xmalloc(sizeof(msa_u32x4) * (size_t)(nref * R), "refresh counters");
+    msa_lut *luts = (msa_lut *)xmalloc(sizeof(msa_lut) * 2 * (size_t)nref, "refresh tables");
+    int32_t *lut_idx = (int32_t *)xmalloc(sizeof(int32_t) * (size_t)nref, "refresh indices");
+    msa_sched_tables_resume(&sch, k_start, nref, R, pr, xo, (uint32_t)sample, ctr, luts, lut_idx);
+    CHK(msa_set_luts(ctx, (int32_t)nref, luts));
+    CHK(msa_set_refresh(ctx, (int32_t)nref, ctr, keys, lut_idx));
+    free(luts);
+    free(ctr);
+    free(lut_idx);
+  }
 
   int32_t *unsat = (int32_t *)calloc((size_t)R * 64, sizeof(int32_t));
   int32_t *demon = (int32_t *)calloc((size_t)R * 64, sizeof(int32_t));
@@ -760,9 +822,19 @@ int main(int argc, char **argv)
       msa_xo256 gxo[MSA_MAXR];
       memcpy(gpr, base_pr, sizeof gpr);
       memcpy(gxo, base_xo, sizeof gxo);
-      msa_gen_skip(gpr, gxo, R, msa_sched_count(&sch, (int64_t)iter) - k_start);
+      if (refmode) {
+        /* the thermal generators are not drawn after the thermalization; the
+         * walker generators are the streams after the last refresh */
+        msa_walker_stream *st = (msa_walker_stream *)xmalloc(
+            sizeof(msa_walker_stream) * nbits * R, "walker streams");
+        CHK(msa_get_walker_streams(ctx, st));
+        for (int k = 0; k < nbits * R; k++) msa_walker_stream_to(&st[k], rf.walker_p, &wpr[k], &wxo[k]);
+        free(st);
+      } else {
+        msa_gen_skip(gpr, gxo, R, msa_sched_count(&sch, (int64_t)iter) - k_start);
+      }
       write_checkpoint(&rf, iter, it > 0 ? times[it - 1] : 0, flag, seed_u, seed_MC, S, D0, D1,
-                       energy, gpr, gxo);
+                       energy, gpr, gxo, wpr, wxo);
     }
     check_signal();
     if (check_time(max_time, dt0, it + 1 < in.ntimes ? times[it + 1] - iter : 0) ||
@@ -784,5 +856,7 @@ int main(int argc, char **argv)
   free(S);
   free(D0);
   free(D1);
+  free(wpr);
+  free(wxo);
   return 0;
 }

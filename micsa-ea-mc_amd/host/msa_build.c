@@ -49,3 +49,33 @@ void msa_build_spins(int L, int R, uint64_t seed_u, uint64_t *spin)
       spin[(uint64_t)r * V + s] = w;
     }
 }
+
+/* The JANUS build's Init_J with the override of io.c:797-799: every sample
+ * uses one seed, so every lane of a J word is the same bit. */
+void msa_build_couplings_one(int L, uint64_t seed, uint64_t *J)
+{
+  const uint64_t V = (uint64_t)L * L * L;
+  msa_prng g;
+  msa_janus_seed(&g, seed);
+  for (uint64_t s = 0; s < V; s++)
+    for (int d = 0; d < 3; d++) J[d * V + s] = (msa_janus_next(&g) >> 63) ? ~0ull : 0ull;
+}
+
+/* Init_u (ini.c:275-299) with `nslot` sample slots per site (MAXNSAMPLES,
+ * header.h:20): one Init_Rand_HQ_64bits(seed_u) stream drawn in (replica,
+ * site, slot) order; lane b < nslot is slot b, the other lanes are zero. */
+void msa_build_spins_slots(int L, int R, uint64_t seed_u, int nslot, uint64_t *spin)
+{
+  const uint64_t V = (uint64_t)L * L * L;
+  msa_prng g;
+  msa_prng_seed_hq(&g, seed_u);
+  for (int r = 0; r < R; r++)
+    for (uint64_t s = 0; s < V; s++) {
+      uint64_t w = 0;
+      for (int b = 0; b < nslot; b++) {
+        const uint64_t v = msa_prng_next(&g) >> 63;
+        if (b < 64) w |= v << b;
+      }
+      spin[(uint64_t)r * V + s] = w;
+    }
+}

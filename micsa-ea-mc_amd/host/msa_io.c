@@ -147,7 +147,8 @@ int msa_read_schedule(const char *path, msa_sched *s, char *e, int elen)
   int ok = fscanf(f, "%lf %lf %lld %lld", &s->T0, &s->T1, &n, &every) == 4;
   fclose(f);
   if (!ok) return err(e, elen, "%s needs T0 T1 nsweeps refresh_every", path);
-  if (s->T0 <= 0 || s->T1 <= 0 || n <= 0 || (every <= 0 && every != -1))
+  const int fixed = s->T0 == 0 && s->T1 == 0;
+  if ((!fixed && (s->T0 <= 0 || s->T1 <= 0)) || n <= 0 || (every <= 0 && every != -1))
     return err(e, elen, "bad schedule in %s", path);
   s->nsweeps = n;
   s->refresh_every = every;

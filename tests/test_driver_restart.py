@@ -22,6 +22,14 @@ def sample_dir(out, sample=3):
     return os.path.join(out, "I%03d" % sample)
 
 
+def fixed_schedule(d, maxtime=16):
+    """Arguments selecting the Philox-refresh run at beta.dat (T0 = T1 = 0)."""
+    path = os.path.join(d, "sched.in")
+    with open(path, "w") as f:
+        f.write("0\n0\n%d\n-1\n" % maxtime)
+    return ["0", "", path]
+
+
 # ------------------------------------------------------------- CPU checks --
 def make_checkpoint(pkg, out, L=8, R=2, nc=1, lanes=(0, 1), now=4, sample=3):
     """A consistent set of seeds, conf and walkers files for the driver's
@@ -85,7 +93,7 @@ def test_new_run_refuses_existing_confs(tmp_path, pkg):
     d = str(tmp_path)
     out = write_inputs(d, flag=1)
     make_checkpoint(pkg, out)
-    p = run_driver(d)
+    p = run_driver(d, fixed_schedule(d))
     assert p.returncode == 1 and "There are some confs" in p.stderr
     assert "There are some confs" in open(os.path.join(sample_dir(out), "error")).read()
 
@@ -95,7 +103,7 @@ def test_missing_replica_conf(tmp_path, pkg):
     out = write_inputs(d, flag=0)
     sd = make_checkpoint(pkg, out)
     os.remove(os.path.join(sd, "BIT001", "R01", "conf"))
-    p = run_driver(d)
+    p = run_driver(d, fixed_schedule(d))
     assert p.returncode == 1 and "less conf files" in p.stderr
 
 
@@ -103,7 +111,7 @@ def test_seed_file_disagrees_with_input(tmp_path, pkg):
     d = str(tmp_path)
     out = write_inputs(d, flag=0, seeds=(1002,) + SEEDS[1:])
     make_checkpoint(pkg, out)
-    p = run_driver(d)
+    p = run_driver(d, fixed_schedule(d))
     assert p.returncode == 1 and "Master seedJ in input file" in p.stderr
 
 
@@ -113,7 +121,7 @@ def test_good_checkpoint_passes_the_checks(tmp_path, pkg):
     d = str(tmp_path)
     out = write_inputs(d, flag=2)
     make_checkpoint(pkg, out)
-    p = run_driver(d)
+    p = run_driver(d, fixed_schedule(d))
     if p.returncode == 0:
         pytest.skip("a GPU is present: the restart ran")
     assert "msa_create" in p.stderr, p.stderr
@@ -129,7 +137,7 @@ def test_corrupt_conf_energy(tmp_path, pkg):
     raw[2321 + 10] ^= 0x04                             # one spin of the payload
     os.remove(path)
     open(path, "wb").write(raw)
-    p = run_driver(d)
+    p = run_driver(d, fixed_schedule(d))
     assert p.returncode == 1 and "Energy read and calculate differ in ibit=1 ir=1" in p.stderr
 
 
@@ -141,7 +149,7 @@ def test_truncated_walkers(tmp_path, pkg):
     raw = open(path, "rb").read()
     os.remove(path)
     open(path, "wb").write(raw[:-8])
-    p = run_driver(d)
+    p = run_driver(d, fixed_schedule(d))
     assert p.returncode == 1 and "bytes in" in p.stderr
 
 
@@ -153,7 +161,7 @@ def test_countJ_mismatch(tmp_path, pkg):
     f = open(path).read().split()
     f[-1] = str(int(f[-1]) + 1)
     open(path, "w").write(" ".join(f) + "\n")
-    p = run_driver(d)
+    p = run_driver(d, fixed_schedule(d))
     assert p.returncode == 1 and "countJ and countJ0 differ" in p.stderr
 
 
@@ -180,8 +188,11 @@ def assert_same_outputs(outA, outB, lanes, R, sample=3):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("nterm,sched", [(0, None), (3, None), (0, "2.0\n0.7\n256\n4\n")])
+@pytest.mark.parametrize("nterm,sched", [(0, None), (3, "0\n0\n256\n-1\n"),
+                                         (0, "2.0\n0.7\n256\n4\n")])
 def test_restart_after_maxtime_continues_bit_for_bit(tmp_path, nterm, sched):
+    """The reference's own workload (no schedule file, walker streams), and
+    the Philox-refresh runs at fixed beta with thermalization and annealed."""
     L, R, nc = 16, 2, 2
     times = tuple(2 ** k for k in range(9))            # 1 .. 256
     extra = []
@@ -262,3 +273,17 @@ def test_sigterm_then_restart(tmp_path, long_reference):
     p = run_driver(d)
     assert p.returncode == 0 and "finished at iteration %d" % LONG["maxtime"] in p.stdout, p.stderr
     assert_same_outputs(long_reference, out, (0, 1), LONG["nr"])
+
+
+@pytest.mark.gpu
+def test_reference_workload_restart_repeats_thermalization(tmp_path):
+    """As the reference (main.cu:124-128), a restart of its own workload runs
+    the thermalization again on the restored state."""
+    d = str(tmp_path)
+    write_inputs(d, L=16, nr=2, maxtime=32, times=(1, 2, 4, 8, 16, 32), nterm=2)
+    p = run_driver(d)
+    assert p.returncode == 0, p.stderr
+    write_inputs(d, L=16, nr=2, maxtime=64, times=(1, 2, 4, 8, 16, 32, 64), nterm=2, flag=2)
+    p = run_driver(d)
+    assert p.returncode == 0 and "finished at iteration 64" in p.stdout, p.stderr
+    assert "Thermalization: 2 Metropolis iterations" in p.stdout
