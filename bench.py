@@ -264,6 +264,10 @@ def main():
                     help="arrival rate as a fraction of the measured batched capacity")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the library's default tiling instead of autotuning it on this lattice")
+    ap.add_argument("--walker-refreshes", type=int, default=0,
+                    help="also time this many refreshes from the reference's walker streams")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="also time a streaming copy of this many MiB (bandwidth calibration)")
     ap.add_argument("--tiling", type=str, default="",
                     help="MODE,TY,TZ: use this sweep kernel and tile (no autotune)")
     ap.add_argument("--thermal-iters", type=int, default=0,
@@ -322,7 +326,26 @@ def main():
         thermal = {"iterations": args.thermal_iters, "beta": round(1.0 / 1.8, 6),
                    "ms_per_iteration": round((h1 - h0) * 1e3 / args.thermal_iters, 4),
                    "spin_updates_per_ps": round(64.0 * R * V * args.thermal_iters / (h1 - h0) / 1e12, 4)}
+    walkers = None
+    if args.walker_refreshes > 0 and L % 8 == 0:
+        # the reference's own demon refresh (two samples, its walker streams):
+        # iterations 2^k refresh, so each timed call is one refresh + one sweep
+        streams, _ = pkg.walker_streams(2002, 2, R)
+        lat.set_walker_streams([0, 1], streams,
+                               np.tile(pkg.thresholds(1.0 / 1.8), (args.walker_refreshes + 2, 1)))
+        lat.run(0, 1, -1)
+        both = [lat.run_timed(1 << k, 1, -1) for k in range(args.walker_refreshes)]
+        plain = lat.run_timed(3, 1, -1)
+        lat.set_walker_streams([], [], np.zeros((1, 3), np.uint64))
+        walkers = {"refreshes": args.walker_refreshes, "samples": 2,
+                   "ms_per_refresh": round(float(np.mean(both)) - plain, 4),
+                   "ms_sweep_alone": round(plain, 4)}
     lat.close()
+    copy = None
+    if args.copy_probe_mb > 0:
+        ms_c = pkg.copy_probe(args.copy_probe_mb << 20, 10)
+        copy = {"bytes": args.copy_probe_mb << 20,
+                "GBs": round(2.0 * (args.copy_probe_mb << 20) / (ms_c * 1e-3) / 1e9, 1)}
 
     t0 = args.warmup
     nref = sum(1 for t in range(t0, t0 + args.steps) if every > 0 and t % every == 0)
@@ -382,6 +405,10 @@ def main():
         out["autotune_ms_per_sweep"] = round(tuned[3], 5)
     if thermal is not None:
         out["thermal"] = thermal
+    if walkers is not None:
+        out["walker_refresh"] = walkers
+    if copy is not None:
+        out["copy_probe"] = copy
     if args.batched > 0:
         out["batched"] = batched_vs_plain(pkg, args.batched, args.batched_len_div)
     if args.served > 0:
