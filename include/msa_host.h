@@ -183,6 +183,20 @@ void msa_walker_generators(uint64_t seed_u, int n, int R, msa_prng *pr, msa_xo25
 void msa_walker_stream_of(const msa_prng *pr, const msa_xo256 *xo, msa_walker_stream *st);
 void msa_walker_stream_to(const msa_walker_stream *st, uint8_t p, msa_prng *pr, msa_xo256 *xo);
 
+/* ------------------------------------------------- offline observables --
+ * From the spins of conf files (two replicas of one sample): q_x = s^a s^b,
+ * C4(r) = <q_x q_{x + r e_mu}> over sites and the three axes, and the
+ * integral correlation lengths xi_{k,k+1} = I_{k+1}/I_k, I_k = int r^k C4. */
+
+/* diff[mu*(L/2+1) + r] = sites where q_x != q_{x + r e_mu}, r = 0..L/2 (exact
+ * integers; L a multiple of 8).  Returns 0, -1 on bad L. */
+int  msa_overlap_counts(int L, const uint8_t *janus_a, const uint8_t *janus_b, int64_t *diff);
+/* C4(r), r = 0..L/2, from counts summed over nterms (sample, pair) terms. */
+void msa_c4_from_counts(int L, int64_t nterms, const int64_t *diff, double *c4);
+/* Trapezoid integrals up to rc = the last r before C4 <= 0 (at most nr-1);
+ * returns rc. */
+int  msa_xi_integrals(int nr, const double *c4, double *xi12, double *xi23);
+
 /* -------------------------------------------------------- checkpoints --
  * The reference's checkpoint files (io.c:17-393) with NBETAS = 1 and NR = R
  * replicas, byte for byte; L must be a multiple of 8 (its multispin layout).

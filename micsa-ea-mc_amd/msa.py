@@ -358,6 +358,11 @@ def host():
         L.msa_msc_to_lanes.argtypes = [C.c_int, C.c_int, C.c_int, u64p, C.c_int, i32p, u64p, u64p]
         L.msa_lane_energy.argtypes = [C.c_int, C.c_int, u64p, u64p, i32p]
         L.msa_lane_countJ.argtypes = [C.c_int, u64p, i32p]
+        i64p = C.POINTER(C.c_int64)
+        L.msa_overlap_counts.argtypes = [C.c_int, u8p, u8p, i64p]
+        L.msa_c4_from_counts.argtypes = [C.c_int, C.c_int64, i64p, C.POINTER(C.c_double)]
+        L.msa_xi_integrals.argtypes = [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                       C.POINTER(C.c_double)]
         _host = L
     return _host
 
@@ -680,3 +685,29 @@ def thresholds(beta):
     t = np.zeros(3, np.uint64)
     host().msa_thresholds(beta, _p(t))
     return t
+
+
+# ------------------------------------------------------ offline observables --
+def overlap_counts(L, janus_a, janus_b):
+    """int64 [3][L/2+1]: sites where q_x != q_{x + r e_mu}."""
+    d = np.zeros((3, L // 2 + 1), np.int64)
+    a = np.ascontiguousarray(janus_a, np.uint8)
+    b = np.ascontiguousarray(janus_b, np.uint8)
+    if host().msa_overlap_counts(L, _p(a, _u8p), _p(b, _u8p), _p(d, C.POINTER(C.c_int64))):
+        raise MsaError("overlap counts: bad L=%d" % L)
+    return d
+
+
+def c4_from_counts(L, nterms, diff):
+    c4 = np.zeros(L // 2 + 1, np.float64)
+    d = np.ascontiguousarray(diff, np.int64)
+    host().msa_c4_from_counts(L, nterms, _p(d, C.POINTER(C.c_int64)), _p(c4, C.POINTER(C.c_double)))
+    return c4
+
+
+def xi_integrals(c4):
+    """(xi12, xi23, rc) of a correlation function c4[r], r = 0..nr-1."""
+    c = np.ascontiguousarray(c4, np.float64)
+    x12, x23 = C.c_double(), C.c_double()
+    rc = host().msa_xi_integrals(c.size, _p(c, C.POINTER(C.c_double)), C.byref(x12), C.byref(x23))
+    return x12.value, x23.value, rc
