@@ -399,7 +399,7 @@ static void batch_launch(msa_batch *b, int col) {
   }
 }
 
-// The batch kernels stage two search tables (37 KiB) in dynamic LDS.
+// The batch kernels stage two search tables (68 KiB) in dynamic LDS.
 static int batch_lds_attr() {
   const void *fns[] = {reinterpret_cast<const void *>(&k_batch_halfsweep<128>),
                        reinterpret_cast<const void *>(&k_batch_halfsweep<256>),

@@ -102,10 +102,11 @@ __device__ __forceinline__ void philox10(uint32_t &v0, uint32_t &v1, uint32_t &v
 // reference's own presearch has 2^10 buckets of (min, max) pairs
 // (NUMBITSPREBUSQUEDAS = 10); this one is built on the device from the same
 // thresholds (k_fine_luts), finer so that almost every search ends in its
-// bucket, and half the size per bucket so two tables stage into LDS cheaply.
+// bucket (a bracket of width 0, no threshold read, for all but ~1 % of R), and
+// half the size per bucket so two tables (2 x 34 KiB) stage into LDS.
 // The byte found is the same for every R: the unique n with
 // umbral[n-1] <= R < umbral[n].
-constexpr int kPreBits = 14;
+constexpr int kPreBits = 15;
 struct FineLut {
   uint32_t umb[512];                       // 256 x (msb32, lsb32)
   uint8_t pre[(1u << kPreBits) + 16];      // + the sentinel 255, padded
@@ -132,6 +133,7 @@ __device__ __forceinline__ uint32_t bisect(const LutView &t, uint32_t rm, uint32
   };
   const uint32_t b = rm >> (32 - kPreBits);
   uint32_t lo = t.pre[b], hi = t.pre[b + 1];
+  if (lo == hi) return lo;                 // no threshold inside the bucket
   // at most 8 halvings of a bracket inside 0..255
   for (int it = 0; it < 9 && hi - lo > 1; ++it) {
     const uint32_t mid = (hi + lo) >> 1;

@@ -65,7 +65,7 @@ __device__ __forceinline__ uint32_t div_recip(uint32_t n, uint32_t m) { return _
 // the same bit for every sample, as rand_h is indexed in main.cu:877-879.
 // Thread = one site of the colour, all R replicas (J loaded once).  Blocks
 // are persistent: each stages the two search tables of the refresh in LDS
-// once (2 x 18 KiB) and then walks tiles of nr rows.  Used for every sweep
+// once (2 x 34 KiB) and then walks tiles of nr rows.  Used for every sweep
 // with a refresh (VALU-bound: Philox and the table search), for the
 // two-pass mode and for thermal sweeps.
 template <int KIND>
@@ -948,7 +948,7 @@ extern "C" int msa_create(int device, const msa_geom *geom, msa_ctx **out) {
       hipMemsetAsync(c->jp, 0, 8 * V * 3, c->stream) != hipSuccess ||
       hipStreamSynchronize(c->stream) != hipSuccess)
     return cleanup(msa_fail(MSA_ERR_HIP, "initial clear failed"));
-  // the refresh kernel stages two search tables (37 KiB) in LDS; the
+  // the refresh kernel stages two search tables (68 KiB) in LDS; the
   // sweep-pair kernel's buffers take up to a CU's whole LDS
   if (hipFuncSetAttribute(reinterpret_cast<const void *>(&k_halfsweep<kRefresh>),
                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)kRefreshLds) != hipSuccess ||
