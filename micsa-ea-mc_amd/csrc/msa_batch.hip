@@ -2,9 +2,9 @@
 // the same kernel launches (see msa_batch_* in include/msa.h).
 //
 // Every lattice is a full msa_ctx (its own ping-pong state and refresh
-// tables); the batch adds a device array of lattice descriptors, a work list
-// of (lattice, tile) items covering every active lattice, and a captured HIP
-// graph of `chunk` iterations (two launches each: colour 0, colour 1).  Each
+// tables); the batch adds a device array of lattice descriptors, a split of
+// the tiles of every active lattice over persistent blocks, and a captured
+// HIP graph of `chunk` iterations (two launches each: colour 0, colour 1).  Each
 // lattice keeps its iteration counter t on the device; the last block of a
 // lattice to finish colour 1 advances it, so the captured graph can be
 // replayed any number of times and lattices that reach their length stop by
@@ -27,15 +27,25 @@ struct BLat {
   const msa::FineLut *luts;
   const int32_t *lut_idx;
   int64_t every;
-  uint64_t t, tend;
-  uint32_t nitems, done;  // work items of this lattice; blocks finished this iteration
+  uint64_t t, tend;       // t: the host's copy (the kernels use BProg)
+  uint32_t ntiles;        // tiles of this lattice
   int32_t nref;
   uint32_t bx, nr, nxc;   // a tile is bx site pairs of nr rows (y, z); nxc tiles per row block
   uint32_t mLh, mL;       // reciprocals for n / Lh and n / L (div_recip)
 };
 
-// A block's work: tiles tile0 .. tile0+ntile-1 of lattice `lat`.
-struct BItem {
+// The progress of a lattice, the only batch state the kernels write: its
+// iteration counter and the tiles of colour 1 finished in this iteration.
+// Kept apart from the read-only descriptors so that those load into scalar
+// registers.
+struct BProg {
+  uint64_t t;
+  uint32_t done, pad;
+};
+
+// A run of tiles tile0 .. tile0+ntile-1 of lattice `lat`.  Block b of a
+// launch works through segments seg_start[b] .. seg_start[b+1]-1.
+struct BSeg {
   uint32_t lat, tile0, ntile, pad;
 };
 
@@ -44,98 +54,128 @@ struct BItem {
 static inline uint32_t div_recip_of(uint32_t d) { return (uint32_t)((1ull << 32) / d + 1); }
 __device__ __forceinline__ uint32_t div_recip(uint32_t n, uint32_t m) { return __umulhi(n, m); }
 
-// One colour of one sweep for every work item: the update of k_halfsweep
-// (d_oneMCstepBN_multisample + fused walk + fused refresh) for nc = 1, with
-// the lattice's own geometry, buffers, counters and tables.  A block stages
-// the lattice's two search tables in LDS once (when its iteration refreshes)
-// and then updates its tiles.  Index arithmetic is 32-bit (every batched
-// lattice has L^3 < 2^31, checked on admission) and divisions by L/2 and L
-// use per-lattice reciprocals.
+// One colour of one sweep for every lattice of the batch: the update of
+// k_halfsweep (d_oneMCstepBN_multisample + fused walk + fused refresh) for
+// nc = 1, with each lattice's own geometry, buffers, counters and tables.
+// Blocks are persistent: the host splits the tiles of all unfinished lattices
+// into one contiguous run per block (consecutive runs on one XCD), so a
+// block's run covers one lattice or the end of one and the start of the next;
+// it stages a lattice's two search tables in LDS when its run reaches that
+// lattice and the lattice refreshes this iteration.  Index arithmetic is
+// 32-bit (every batched lattice has L^3 < 2^31, checked on admission) and
+// divisions by L/2 and L use per-lattice reciprocals.
 template <int BS>
-__global__ void __launch_bounds__(BS) k_batch_halfsweep(BLat *lats, const BItem *items, int colour) {
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) k_batch_halfsweep(const BLat *__restrict__ lats, BProg *prog,
+                                                        const BSeg *__restrict__ segs,
+                                                        const uint32_t *__restrict__ seg_start,
+                                                        int colour) {
   extern __shared__ uint4 s_tab[];
-  const BItem it = items[blockIdx.x];
-  BLat *lp = lats + it.lat;
-  const uint64_t t = lp->t;
-  if (t >= lp->tend) return;                     // whole block: lattice finished
-  const msa::Geo g = lp->g;
-  const int64_t k = refresh_index(t, lp->every);
-  const bool refresh = k >= 0 && k < lp->nref;
   const msa::FineLut *tabs = reinterpret_cast<const msa::FineLut *>(s_tab);
-  if (refresh) {
-    const uint4 *src = reinterpret_cast<const uint4 *>(lp->luts + 2 * (size_t)lp->lut_idx[k]);
-    const uint32_t n16 = 2 * sizeof(msa::FineLut) / 16;
-    for (uint32_t j = threadIdx.x; j < n16; j += BS) s_tab[j] = src[j];
-    __syncthreads();
-  }
   const msa::LutView t0 = msa::lut_view(tabs), t1 = msa::lut_view(tabs + 1);
-  const int par = (int)(t & 1u);
-  const uint64_t *A = lp->sp[par];
-  uint64_t *B = lp->sp[par ^ 1];
-  const ulonglong2 *dmi = lp->dm[par];
-  ulonglong2 *dmo = lp->dm[par ^ 1];
-  const uint32_t bx = lp->bx, nr = lp->nr, nxc = lp->nxc, Lh = g.Lh, L = g.L;
-  const uint32_t rloc = bx == Lh ? div_recip(threadIdx.x, lp->mLh) : threadIdx.x / bx;
-  const uint32_t xin = threadIdx.x - rloc * bx;
-  const int col = colour;
-  const uint32_t H = (uint32_t)g.H;
-  const uint64_t *Jc = lp->jp + (uint64_t)col * 3 * g.H;
-  const uint64_t *Jo = lp->jp + (uint64_t)(col ^ 1) * 3 * g.H;
-  const uint64_t *Sn = col ? B : A;
-  for (uint32_t tt = 0; tt < it.ntile; ++tt) {
-    const uint32_t tile = it.tile0 + tt;
-    const uint32_t rbk = nxc == 1 ? tile : tile / nxc;
-    const uint32_t i = (tile - rbk * nxc) * bx + xin;
-    const uint32_t row = rbk * nr + rloc;
-    if (rloc >= nr || i >= Lh || row >= L * L) continue;
-    const uint32_t z = div_recip(row, lp->mL), y = row - z * L;
-    const uint32_t off = (y + z + (uint32_t)col) & 1u;
-    const uint32_t x = 2u * i + off;
-    const uint32_t ip = off ? msa::wrapi((int)i + 1, Lh) : i;
-    const uint32_t im = off ? i : msa::wrapi((int)i - 1, Lh);
-    const uint32_t yp = msa::wrapi((int)y + 1, L), ym = msa::wrapi((int)y - 1, L);
-    const uint32_t zp = msa::wrapi((int)z + 1, L), zm = msa::wrapi((int)z - 1, L);
-    const uint32_t rb = Lh * row;
-    const uint32_t h = i + rb;
-    const uint32_t hxp = ip + rb, hxm = im + rb;
-    const uint32_t hyp = i + Lh * (yp + L * z), hym = i + Lh * (ym + L * z);
-    const uint32_t hzp = i + Lh * (y + L * zp), hzm = i + Lh * (y + L * zm);
-    const uint32_t hw = (uint32_t)msa::walk_dest(g, col, 1, 0, 0, i, y, z);   // layer 0 walks along +x
-    const uint32_t tid = x + L * row;
-    uint64_t J0, J1, J2, J3, J4, J5;
-    for (uint32_t r = 0; r < g.R; ++r) {
-      const uint64_t *So = Sn + (uint64_t)(2 * r + (col ^ 1)) * g.H;
-      const ulonglong2 *Di = dmi + ((uint64_t)r * 2 + col) * g.H;
-      ulonglong2 *Do = dmo + ((uint64_t)r * 2 + col) * g.H;
-      uint64_t D0, D1;
-      if (refresh) {
-        // the refresh first: its temporaries are dead before J and the
-        // neighbours are loaded
-        msa::refresh_demon(t0, t1, lp->ctr[(size_t)k * g.R + r], lp->key[r], tid, D0, D1);
-      } else {
-        const ulonglong2 d = Di[h];
-        D0 = d.x;
-        D1 = d.y;
+  const msa::FineLut *staged = nullptr;
+  const uint32_t s_end = seg_start[blockIdx.x + 1];
+  for (uint32_t s = seg_start[blockIdx.x]; s < s_end; ++s) {
+    const BSeg it = segs[s];
+    const BLat *lp = lats + it.lat;
+    BProg *pp = prog + it.lat;
+    // t moves only once every tile of the lattice is counted, this run's
+    // included, so every thread of the block reads the same value
+    const uint64_t tv = pp->t;
+    const uint64_t t = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(tv >> 32)) << 32) |
+                       __builtin_amdgcn_readfirstlane((uint32_t)tv);
+    if (t >= lp->tend) continue;                   // whole block: lattice finished
+    const msa::Geo g = lp->g;
+    const int64_t k = refresh_index(t, lp->every);
+    const bool refresh = k >= 0 && k < lp->nref;
+    if (refresh) {
+      const msa::FineLut *src = lp->luts + 2 * (size_t)lp->lut_idx[k];
+      if (src != staged) {
+        __syncthreads();                           // the previous tables are no longer read
+        const uint4 *s16 = reinterpret_cast<const uint4 *>(src);
+        const uint32_t n16 = 2 * sizeof(msa::FineLut) / 16;
+        for (uint32_t j = threadIdx.x; j < n16; j += BS) s_tab[j] = s16[j];
+        __syncthreads();
+        staged = src;
       }
-      if (r == 0) {
-        J0 = Jc[h]; J1 = Jc[H + h]; J2 = Jc[2 * H + h];
-        J3 = Jo[hxm]; J4 = Jo[H + hym]; J5 = Jo[2 * H + hzm];
-      }
-      const uint64_t sown = A[(uint64_t)(2 * r + col) * g.H + h];
-      const uint64_t sn = msa::site_update(sown, So[hxp], So[hyp], So[hzp], So[hxm], So[hym],
-                                           So[hzm], J0, J1, J2, J3, J4, J5, D0, D1);
-      B[(uint64_t)(2 * r + col) * g.H + h] = sn;
-      Do[hw] = make_ulonglong2(D0, D1);
     }
-  }
-  if (colour == 1) {
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      __threadfence();
-      if (atomicAdd(&lp->done, 1u) == lp->nitems - 1) {   // last block of this lattice
-        lp->done = 0;
-        lp->t = t + 1;
+    const int par = (int)(t & 1u);
+    const uint64_t *A = lp->sp[par];
+    uint64_t *B = lp->sp[par ^ 1];
+    const ulonglong2 *dmi = lp->dm[par];
+    ulonglong2 *dmo = lp->dm[par ^ 1];
+    const uint32_t bx = lp->bx, nr = lp->nr, nxc = lp->nxc, Lh = g.Lh, L = g.L;
+    const uint32_t rloc = bx == Lh ? div_recip(threadIdx.x, lp->mLh) : threadIdx.x / bx;
+    const uint32_t xin = threadIdx.x - rloc * bx;
+    const int col = colour;
+    const uint32_t H = (uint32_t)g.H;
+    const uint64_t *Jc = lp->jp + (uint64_t)col * 3 * g.H;
+    const uint64_t *Jo = lp->jp + (uint64_t)(col ^ 1) * 3 * g.H;
+    const uint64_t *Sn = col ? B : A;
+    const uint4 *ctr = lp->ctr + (refresh ? (size_t)k * g.R : 0);
+    for (uint32_t tt = 0; tt < it.ntile; ++tt) {
+      const uint32_t tile = it.tile0 + tt;
+      const uint32_t rbk = nxc == 1 ? tile : tile / nxc;
+      const uint32_t i = (tile - rbk * nxc) * bx + xin;
+      const uint32_t row = rbk * nr + rloc;
+      if (rloc >= nr || i >= Lh || row >= L * L) continue;
+      const uint32_t z = div_recip(row, lp->mL), y = row - z * L;
+      const uint32_t off = (y + z + (uint32_t)col) & 1u;
+      const uint32_t x = 2u * i + off;
+      const uint32_t ip = off ? msa::wrapi((int)i + 1, Lh) : i;
+      const uint32_t im = off ? i : msa::wrapi((int)i - 1, Lh);
+      const uint32_t yp = msa::wrapi((int)y + 1, L), ym = msa::wrapi((int)y - 1, L);
+      const uint32_t zp = msa::wrapi((int)z + 1, L), zm = msa::wrapi((int)z - 1, L);
+      const uint32_t rb = Lh * row;
+      const uint32_t h = i + rb;
+      const uint32_t hxp = ip + rb, hxm = im + rb;
+      const uint32_t hyp = i + Lh * (yp + L * z), hym = i + Lh * (ym + L * z);
+      const uint32_t hzp = i + Lh * (y + L * zp), hzm = i + Lh * (y + L * zm);
+      const uint32_t hw = (uint32_t)msa::walk_dest(g, col, 1, 0, 0, i, y, z);   // layer 0 walks along +x
+      const uint32_t tid = x + L * row;
+      uint64_t J0, J1, J2, J3, J4, J5;
+      for (uint32_t r = 0; r < g.R; ++r) {
+        const uint64_t *So = Sn + (uint64_t)(2 * r + (col ^ 1)) * g.H;
+        const ulonglong2 *Di = dmi + ((uint64_t)r * 2 + col) * g.H;
+        ulonglong2 *Do = dmo + ((uint64_t)r * 2 + col) * g.H;
+        uint64_t sown, n0, n1, n2, n3, n4, n5;
+        auto load = [&]() {
+          if (r == 0) {
+            J0 = Jc[h]; J1 = Jc[H + h]; J2 = Jc[2 * H + h];
+            J3 = Jo[hxm]; J4 = Jo[H + hym]; J5 = Jo[2 * H + hzm];
+          }
+          sown = A[(uint64_t)(2 * r + col) * g.H + h];
+          n0 = So[hxp]; n1 = So[hyp]; n2 = So[hzp]; n3 = So[hxm]; n4 = So[hym]; n5 = So[hzm];
+        };
+        uint64_t D0, D1;
+        if (refresh) {
+          // the site's loads are issued first and land while the refresh
+          // (Philox and the table search, VALU-bound) runs
+          load();
+          const uint4 c = ctr[r];
+          const uint32_t key = lp->key[r];
+          D0 = msa::refresh_plane(t0, c, key, tid, 0);
+          D1 = msa::refresh_plane(t1, c, key, tid, 1);
+        } else {
+          const ulonglong2 d = Di[h];
+          D0 = d.x;
+          D1 = d.y;
+          load();
+        }
+        const uint64_t sn = msa::site_update(sown, n0, n1, n2, n3, n4, n5, J0, J1, J2, J3, J4, J5,
+                                             D0, D1);
+        B[(uint64_t)(2 * r + col) * g.H + h] = sn;
+        Do[hw] = make_ulonglong2(D0, D1);
+      }
+    }
+    if (colour == 1) {
+      __syncthreads();
+      if (threadIdx.x == 0) {
         __threadfence();
+        if (atomicAdd(&pp->done, it.ntile) + it.ntile == lp->ntiles) {   // the lattice's last run
+          pp->done = 0;
+          pp->t = t + 1;
+          __threadfence();
+        }
       }
     }
   }
@@ -164,9 +204,11 @@ struct msa_batch {
   std::vector<BLat> h_lats;
   std::vector<char> fresh;              // added since the device table was last pushed
   BLat *d_lats = nullptr;
+  BProg *d_prog = nullptr;
   size_t cap = 0;
-  BItem *d_items = nullptr;
-  uint32_t nitems = 0, items_cap = 0;
+  BSeg *d_segs = nullptr;                // the blocks' runs of tiles
+  uint32_t *d_seg_start = nullptr;      // [nblocks + 1]
+  uint32_t nblocks = 0, segs_cap = 0, starts_cap = 0;
   msa::FineLut *luts = nullptr;         // shared search tables, 2 per pair
   int32_t nlut = 0;
   int block = 256;
@@ -190,10 +232,10 @@ static int batch_clear_graph(msa_batch *b) {
 // was pushed; a slot can be reused, and its new lattice may even get the old
 // one's buffers back from the allocator, so identity is tracked explicitly).
 static int batch_pull(msa_batch *b) {
-  if (!b->d_lats || !b->cap) return MSA_OK;
+  if (!b->d_prog || !b->cap) return MSA_OK;
   HIPCHK(hipStreamSynchronize(b->stream));
-  std::vector<BLat> cur(b->cap);
-  HIPCHK(hipMemcpyAsync(cur.data(), b->d_lats, sizeof(BLat) * b->cap, hipMemcpyDeviceToHost, b->stream));
+  std::vector<BProg> cur(b->cap);
+  HIPCHK(hipMemcpyAsync(cur.data(), b->d_prog, sizeof(BProg) * b->cap, hipMemcpyDeviceToHost, b->stream));
   HIPCHK(hipStreamSynchronize(b->stream));
   for (size_t s = 0; s < std::min(b->slot.size(), b->cap); ++s)
     if (b->slot[s] && !b->fresh[s]) b->h_lats[s].t = cur[s].t;
@@ -236,7 +278,9 @@ extern "C" int msa_batch_destroy(msa_batch *b) {
     (void)hipFree(e.idx);
   }
   (void)hipFree(b->d_lats);
-  (void)hipFree(b->d_items);
+  (void)hipFree(b->d_prog);
+  (void)hipFree(b->d_segs);
+  (void)hipFree(b->d_seg_start);
   (void)hipFree(b->luts);
   (void)hipStreamDestroy(b->stream);
   delete b;
@@ -389,13 +433,17 @@ extern "C" int msa_batch_upload_state(msa_batch *b, int32_t slot, const uint64_t
 }
 
 static void batch_launch(msa_batch *b, int col) {
-  const dim3 grid(b->nitems);
+  const dim3 grid(b->nblocks);
   const size_t lds = kRefreshLds;
+  const BLat *L = b->d_lats;
+  BProg *P = b->d_prog;
+  const BSeg *sg = b->d_segs;
+  const uint32_t *st = b->d_seg_start;
   switch (b->block) {
-    case 128: hipLaunchKernelGGL(k_batch_halfsweep<128>, grid, dim3(128), lds, b->stream, b->d_lats, b->d_items, col); break;
-    case 512: hipLaunchKernelGGL(k_batch_halfsweep<512>, grid, dim3(512), lds, b->stream, b->d_lats, b->d_items, col); break;
-    case 1024: hipLaunchKernelGGL(k_batch_halfsweep<1024>, grid, dim3(1024), lds, b->stream, b->d_lats, b->d_items, col); break;
-    default: hipLaunchKernelGGL(k_batch_halfsweep<256>, grid, dim3(256), lds, b->stream, b->d_lats, b->d_items, col); break;
+    case 128: hipLaunchKernelGGL(k_batch_halfsweep<128>, grid, dim3(128), lds, b->stream, L, P, sg, st, col); break;
+    case 512: hipLaunchKernelGGL(k_batch_halfsweep<512>, grid, dim3(512), lds, b->stream, L, P, sg, st, col); break;
+    case 1024: hipLaunchKernelGGL(k_batch_halfsweep<1024>, grid, dim3(1024), lds, b->stream, L, P, sg, st, col); break;
+    default: hipLaunchKernelGGL(k_batch_halfsweep<256>, grid, dim3(256), lds, b->stream, L, P, sg, st, col); break;
   }
 }
 
@@ -410,14 +458,62 @@ static int batch_lds_attr() {
   return MSA_OK;
 }
 
-// Push descriptors and the work list; (re)capture the graph.
+// Blocks of the batch kernel with `block` threads that one CU holds at once
+// (registers and the tables' LDS), at least 1.
+static int batch_resident(int block) {
+  const void *f = reinterpret_cast<const void *>(&k_batch_halfsweep<256>);
+  if (block == 128) f = reinterpret_cast<const void *>(&k_batch_halfsweep<128>);
+  if (block == 512) f = reinterpret_cast<const void *>(&k_batch_halfsweep<512>);
+  if (block == 1024) f = reinterpret_cast<const void *>(&k_batch_halfsweep<1024>);
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, block, kRefreshLds) != hipSuccess || n < 1) n = 1;
+  return n;
+}
+
+// The work split of one launch: the tiles of all unfinished lattices, in slot
+// order, cut into `nblocks` contiguous runs of equal length (to one tile).
+// Block b runs on XCD b mod kXcd, so the runs are dealt out such that each
+// XCD takes consecutive runs: neighbouring tiles, which read each other's
+// boundary rows, then share an L2.  A run is split where it crosses from one
+// lattice to the next (segments).
+constexpr uint32_t kXcd = 8;
+
+static void batch_split(const std::vector<uint32_t> &tiles, const std::vector<uint32_t> &lat,
+                        uint32_t nblocks, std::vector<BSeg> &segs, std::vector<uint32_t> &start) {
+  uint64_t total = 0;
+  std::vector<uint64_t> first(tiles.size() + 1, 0);   // first global tile of each lattice
+  for (size_t j = 0; j < tiles.size(); ++j) first[j + 1] = first[j] + tiles[j];
+  total = first.back();
+  // position of block b in XCD-major order
+  std::vector<uint32_t> rank(nblocks);
+  {
+    uint32_t pos = 0;
+    for (uint32_t x = 0; x < kXcd; ++x)
+      for (uint32_t bb = x; bb < nblocks; bb += kXcd) rank[bb] = pos++;
+  }
+  segs.clear();
+  start.assign(nblocks + 1, 0);
+  for (uint32_t bb = 0; bb < nblocks; ++bb) {
+    start[bb] = (uint32_t)segs.size();
+    const uint64_t lo = total * rank[bb] / nblocks, hi = total * (rank[bb] + 1) / nblocks;
+    // lattice holding global tile lo
+    size_t j = (size_t)(std::upper_bound(first.begin(), first.end(), lo) - first.begin()) - 1;
+    for (uint64_t g = lo; g < hi; ++j) {
+      const uint64_t end = std::min<uint64_t>(hi, first[j + 1]);
+      if (end > g) segs.push_back(BSeg{lat[j], (uint32_t)(g - first[j]), (uint32_t)(end - g), 0});
+      g = end;
+    }
+  }
+  start[nblocks] = (uint32_t)segs.size();
+}
+
+// Push descriptors and the work split; (re)capture the graph.
 static int batch_prepare(msa_batch *b) {
   if (!b->dirty) return MSA_OK;
   const size_t n = b->slot.size();
   int rc = batch_pull(b);               // keep the progress of lattices already running
   if (rc) return rc;
-  std::vector<BItem> items;
-  std::vector<uint32_t> tiles(n, 0);
+  std::vector<uint32_t> tiles, lat;     // unfinished lattices: tile counts and slots
   uint64_t total = 0;
   for (size_t s = 0; s < n; ++s) {
     if (!b->slot[s]) continue;
@@ -428,48 +524,63 @@ static int batch_prepare(msa_batch *b) {
     L.nxc = (Lh + L.bx - 1) / L.bx;
     L.mLh = div_recip_of(Lh);
     L.mL = div_recip_of(L.g.L);
-    L.done = 0;
-    tiles[s] = ((rows + L.nr - 1) / L.nr) * L.nxc;
-    if (L.t < L.tend) total += tiles[s];
+    L.ntiles = ((rows + L.nr - 1) / L.nr) * L.nxc;
+    if (L.t < L.tend) {
+      tiles.push_back(L.ntiles);
+      lat.push_back((uint32_t)s);
+      total += L.ntiles;
+    }
   }
-  // tiles per item: a block stages the search tables once for several tiles,
-  // up to 16, while the unfinished lattices still give every CU about four
-  // items to balance over
-  const uint64_t want = 4ull * (uint64_t)b->ncu;
-  uint32_t per = (uint32_t)((total + want - 1) / want);
-  per = per < 1 ? 1 : (per > 16 ? 16 : per);
-  for (size_t s = 0; s < n; ++s) {
-    if (!b->slot[s]) continue;
-    BLat &L = b->h_lats[s];
-    uint32_t ni = 0;
-    for (uint32_t t0 = 0; t0 < tiles[s]; t0 += per, ++ni)
-      if (L.t < L.tend) items.push_back(BItem{(uint32_t)s, t0, tiles[s] - t0 < per ? tiles[s] - t0 : per, 0});
-    L.nitems = ni;
-  }
+  // persistent blocks: as many as the CUs hold at once, fewer when there are
+  // fewer tiles
+  const uint64_t resident = (uint64_t)b->ncu * (uint64_t)batch_resident(b->block);
+  const uint32_t nblocks = (uint32_t)std::min<uint64_t>(resident, total);
+  std::vector<BSeg> segs;
+  std::vector<uint32_t> start;
+  if (nblocks) batch_split(tiles, lat, nblocks, segs, start);
   if (n > b->cap) {
     (void)hipFree(b->d_lats);
+    (void)hipFree(b->d_prog);
     b->d_lats = nullptr;
+    b->d_prog = nullptr;
     b->cap = 0;
-    if (hipMalloc(&b->d_lats, sizeof(BLat) * n) != hipSuccess) return msa_fail(MSA_ERR_NOMEM, "descriptors");
+    if (hipMalloc(&b->d_lats, sizeof(BLat) * n) != hipSuccess ||
+        hipMalloc(&b->d_prog, sizeof(BProg) * n) != hipSuccess)
+      return msa_fail(MSA_ERR_NOMEM, "descriptors");
     b->cap = n;
   }
-  if (n) HIPCHK(hipMemcpyAsync(b->d_lats, b->h_lats.data(), sizeof(BLat) * n, hipMemcpyHostToDevice, b->stream));
-  for (size_t s = 0; s < n; ++s) b->fresh[s] = 0;
-  if (items.size() > b->items_cap) {
-    (void)hipFree(b->d_items);
-    b->d_items = nullptr;
-    b->items_cap = 0;
-    if (hipMalloc(&b->d_items, sizeof(BItem) * items.size()) != hipSuccess)
-      return msa_fail(MSA_ERR_NOMEM, "work list");
-    b->items_cap = (uint32_t)items.size();
+  std::vector<BProg> prog(n);
+  for (size_t s = 0; s < n; ++s) prog[s] = BProg{b->h_lats[s].t, 0, 0};
+  if (n) {
+    HIPCHK(hipMemcpyAsync(b->d_lats, b->h_lats.data(), sizeof(BLat) * n, hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipMemcpyAsync(b->d_prog, prog.data(), sizeof(BProg) * n, hipMemcpyHostToDevice, b->stream));
   }
-  if (!items.empty())
-    HIPCHK(hipMemcpyAsync(b->d_items, items.data(), sizeof(BItem) * items.size(), hipMemcpyHostToDevice,
+  for (size_t s = 0; s < n; ++s) b->fresh[s] = 0;
+  if (segs.size() > b->segs_cap) {
+    (void)hipFree(b->d_segs);
+    b->d_segs = nullptr;
+    b->segs_cap = 0;
+    if (hipMalloc(&b->d_segs, sizeof(BSeg) * segs.size()) != hipSuccess)
+      return msa_fail(MSA_ERR_NOMEM, "work split");
+    b->segs_cap = (uint32_t)segs.size();
+  }
+  if (start.size() > b->starts_cap) {
+    (void)hipFree(b->d_seg_start);
+    b->d_seg_start = nullptr;
+    b->starts_cap = 0;
+    if (hipMalloc(&b->d_seg_start, sizeof(uint32_t) * start.size()) != hipSuccess)
+      return msa_fail(MSA_ERR_NOMEM, "work split");
+    b->starts_cap = (uint32_t)start.size();
+  }
+  if (!segs.empty()) {
+    HIPCHK(hipMemcpyAsync(b->d_segs, segs.data(), sizeof(BSeg) * segs.size(), hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipMemcpyAsync(b->d_seg_start, start.data(), sizeof(uint32_t) * start.size(), hipMemcpyHostToDevice,
                           b->stream));
+  }
   HIPCHK(hipStreamSynchronize(b->stream));
-  b->nitems = (uint32_t)items.size();
+  b->nblocks = nblocks;
   batch_clear_graph(b);
-  if (b->nitems) {
+  if (b->nblocks) {
     HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     for (int32_t it = 0; it < b->chunk; ++it)
       for (int col = 0; col < 2; ++col) {
@@ -513,7 +624,7 @@ extern "C" int msa_batch_run(msa_batch *b, int64_t niter) {
       int rc = batch_prepare(b);
       if (rc) return rc;
     }
-    if (!b->nitems) break;
+    if (!b->nblocks) break;
     int64_t step = left;
     for (size_t s = 0; s < b->slot.size(); ++s)
       if (b->slot[s] && b->h_lats[s].t < b->h_lats[s].tend)

@@ -163,19 +163,82 @@ __device__ __forceinline__ uint32_t rnd_word(const LutView &t, uint32_t c[4], ui
   return rot ? ((w << rot) | (w >> (32u - rot))) : w;
 }
 
+// The search of bisect() inside a bracket [lo, hi] known to hold the answer
+// (hi > lo), on the 64-bit value R = (rm:rl) with one 8-byte threshold read
+// per step.
+__device__ __forceinline__ uint32_t bracket_search(const LutView &t, uint32_t rm, uint32_t rl,
+                                                   uint32_t lo, uint32_t hi) {
+  const uint64_t r = ((uint64_t)rm << 32) | rl;
+  auto below = [&](uint32_t n) {
+    const uint2 u = t.umb[n];
+    return r < (((uint64_t)u.x << 32) | u.y);
+  };
+  for (int it = 0; it < 9 && hi - lo > 1; ++it) {
+    const uint32_t mid = (hi + lo) >> 1;
+    if (below(mid)) hi = mid; else lo = mid;
+  }
+  return below(lo) ? lo : hi;
+}
+
+// Two consecutive rnd_word() draws from one table (counter x, then x + 1),
+// the same words rnd_word gives: both Philox calls and their mixing first,
+// then the sixteen presearch reads issued together (one LDS wait for all of
+// them rather than one per byte), then the bracket search of the few bytes
+// whose bucket holds a threshold.
+__device__ __forceinline__ void rnd_word_pair(const LutView &t, uint32_t cx, uint32_t cy,
+                                              uint32_t cz, uint32_t cw, uint32_t k0, uint32_t k1,
+                                              uint32_t &wa, uint32_t &wb) {
+  uint32_t m[8], l[8], rot[2];
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    uint32_t v0 = cx + (uint32_t)j, v1 = cy, v2 = cz, v3 = cw;
+    philox10(v0, v1, v2, v3, k0, k1);
+    const uint32_t a = v0 ^ v2, b = v1 ^ v3;
+    const uint32_t l0 = ((v1 << 17) | (v0 >> 15)) ^ a ^ ((a << 21) | (b >> 11));
+    const uint32_t l1 = ((v0 << 17) | (v1 >> 15)) ^ b ^ (b << 21);
+    const uint32_t l2 = (a << 28) | (b >> 4);
+    const uint32_t l3 = (b << 28) | (a >> 4);
+    rot[j] = (l0 + v1 + l2 + v3) >> 27;
+    m[4 * j] = v0; m[4 * j + 1] = v1; m[4 * j + 2] = v2; m[4 * j + 3] = v3;
+    l[4 * j] = l0; l[4 * j + 1] = l1; l[4 * j + 2] = l2; l[4 * j + 3] = l3;
+  }
+  uint32_t lo[8], hi[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const uint32_t bq = m[q] >> (32 - kPreBits);
+    lo[q] = t.pre[bq];
+    hi[q] = t.pre[bq + 1];
+  }
+#pragma unroll
+  for (int q = 0; q < 8; ++q)
+    if (lo[q] != hi[q]) lo[q] = bracket_search(t, m[q], l[q], lo[q], hi[q]);
+  uint32_t w[2];
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const uint32_t x = lo[4 * j] | (lo[4 * j + 1] << 8) | (lo[4 * j + 2] << 16) | (lo[4 * j + 3] << 24);
+    w[j] = rot[j] ? ((x << rot[j]) | (x >> (32u - rot[j]))) : x;
+  }
+  wa = w[0];
+  wb = w[1];
+}
+
 // Demon words of one (layer, site, replica) from the refresh stream:
 // thread id tid = layer*V + site, counter ctr with .z += tid, key {tid, key},
-// two words from the plane-0 table then two from the plane-1 table.
+// two words from the plane-0 table then two from the plane-1 table (rnd_word
+// four times, the counter's x advancing after each).  refresh_plane(p) is
+// plane p alone (D0 for p = 0, D1 for p = 1), so callers can place other
+// work between the two.
+__device__ __forceinline__ uint64_t refresh_plane(const LutView &t, uint4 ctr, uint32_t key,
+                                                  uint32_t tid, int plane) {
+  uint32_t wa, wb;
+  rnd_word_pair(t, ctr.x + 2u * (uint32_t)plane, ctr.y, ctr.z + tid, ctr.w, tid, key, wa, wb);
+  return ((uint64_t)wa << 32) | wb;
+}
 __device__ __forceinline__ void refresh_demon(const LutView &t0, const LutView &t1, uint4 ctr,
                                               uint32_t key, uint32_t tid, uint64_t &d0,
                                               uint64_t &d1) {
-  uint32_t c[4] = {ctr.x, ctr.y, ctr.z + tid, ctr.w};
-  const uint64_t w0 = rnd_word(t0, c, tid, key);
-  const uint64_t w1 = rnd_word(t0, c, tid, key);
-  const uint64_t w2 = rnd_word(t1, c, tid, key);
-  const uint64_t w3 = rnd_word(t1, c, tid, key);
-  d0 = (w0 << 32) | w1;
-  d1 = (w2 << 32) | w3;
+  d0 = refresh_plane(t0, ctr, key, tid, 0);
+  d1 = refresh_plane(t1, ctr, key, tid, 1);
 }
 
 // ------------------------------------------------------------ the update --

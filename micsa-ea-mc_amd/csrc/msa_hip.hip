@@ -131,23 +131,30 @@ __global__ void __launch_bounds__(1024) k_halfsweep(SweepArgs a) {
     for (uint32_t r = 0; r < g.R; ++r) {
       const uint64_t *So = a.sp_nb + (uint64_t)(2 * r + (col ^ 1)) * g.H;
       const uint64_t dbase = ((uint64_t)(r * g.nc + a.layer) * 2 + col) * g.H;
+      uint64_t sown, n0, n1, n2, n3, n4, n5;
+      auto load = [&]() {
+        if (r == 0) {
+          J0 = Jc[h]; J1 = Jc[H + h]; J2 = Jc[2 * H + h];
+          J3 = Jo[hxm]; J4 = Jo[H + hym]; J5 = Jo[2 * H + hzm];
+        }
+        sown = a.sp_own[(uint64_t)(2 * r + col) * g.H + h];
+        n0 = So[hxp]; n1 = So[hyp]; n2 = So[hzp]; n3 = So[hxm]; n4 = So[hym]; n5 = So[hzm];
+      };
       uint64_t D0, D1;
       if (REFRESH) {
-        // the refresh first: its temporaries are dead before J and the
-        // neighbours are loaded
-        msa::refresh_demon(t0, t1, a.ctr[r], a.key[r], tid, D0, D1);
+        // the site's loads are issued first and land while the refresh
+        // (Philox and the table search, VALU-bound) runs
+        load();
+        D0 = msa::refresh_plane(t0, a.ctr[r], a.key[r], tid, 0);
+        D1 = msa::refresh_plane(t1, a.ctr[r], a.key[r], tid, 1);
       } else {
         const ulonglong2 d = a.dm_in[dbase + h];
         D0 = d.x;
         D1 = d.y;
+        load();
       }
-      if (r == 0) {
-        J0 = Jc[h]; J1 = Jc[H + h]; J2 = Jc[2 * H + h];
-        J3 = Jo[hxm]; J4 = Jo[H + hym]; J5 = Jo[2 * H + hzm];
-      }
-      const uint64_t sown = a.sp_own[(uint64_t)(2 * r + col) * g.H + h];
-      const uint64_t sn = msa::site_update(sown, So[hxp], So[hyp], So[hzp], So[hxm], So[hym],
-                                           So[hzm], J0, J1, J2, J3, J4, J5, D0, D1);
+      const uint64_t sn = msa::site_update(sown, n0, n1, n2, n3, n4, n5, J0, J1, J2, J3, J4, J5,
+                                           D0, D1);
       a.sp_out[(uint64_t)(2 * r + col) * g.H + h] = sn;
       a.dm_out[dbase + hw] = make_ulonglong2(D0, D1);
     }

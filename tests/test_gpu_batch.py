@@ -82,6 +82,47 @@ def test_batch_matches_plain_and_oracle(pkg, orc):
     batch.close()
 
 
+@pytest.mark.parametrize("block", [128, 1024])
+def test_batch_runs_span_lattices(pkg, orc, block):
+    """More tiles than resident blocks: each persistent block's run crosses
+    lattice boundaries and stages a different lattice's tables part-way, and
+    lattices finish at different iterations (the split is rebuilt)."""
+    cases = [(64, 1, 9, 1), (32, 1, 6, 1), (40, 2, 5, 2), (32, 1, 11, 3), (48, 1, 4, 1)]
+    built = [lattice_case(orc, L, R, n, e, 40 + i) for i, (L, R, n, e) in enumerate(cases)]
+    raws, offs, off = [], [], 0
+    for b in built:
+        raws.append(b[3])
+        offs.append(off)
+        off += len(b[5])
+    shared = b"".join(raws)
+    batch = pkg.Batch(shared)
+    batch.set_shape(block, 2)
+    slots, plains = [], []
+    for (L, R, n, e), b, o0 in zip(cases, built, offs):
+        o, ctr, keys, raw, luts, times = b
+        idx = np.arange(len(times), dtype=np.int32) + o0
+        s = batch.add(pkg.Batch.make_job(L, R, 1234, n, e, ctr, keys, idx))
+        batch.upload(s, o.J, o.S, o.D0, o.D1)
+        slots.append(s)
+        p = pkg.Lattice(L, R, 1)
+        p.upload(o.J, o.S, o.D0, o.D1)
+        p.set_luts(shared)
+        p.set_refresh(ctr, keys, idx)
+        p.run(0, n, e)
+        plains.append(p)
+    for step in (3, 1, 5, 2):
+        batch.run(step)
+    batch.sync()
+    t, _ = batch.progress(len(slots))
+    assert list(t) == [c[2] for c in cases]
+    for s, p in zip(slots, plains):
+        for g, w in zip(batch.download(s), p.download()):
+            assert np.array_equal(g, w), "slot %d differs from the plain run" % s
+        assert np.array_equal(batch.measure(s)["energy"], p.measure()["energy"])
+        p.close()
+    batch.close()
+
+
 def test_batch_admission_and_churn(pkg, orc):
     o, ctr, keys, raw, luts, times = lattice_case(orc, 16, 1, 12, 1, 3)
     job = pkg.Batch.make_job(16, 1, 77, 12, 1, ctr, keys, np.arange(len(times), dtype=np.int32))
