@@ -56,9 +56,10 @@ def kernel_name(mode, R):
     return name % R if "%d" in name else name
 
 
-def read_traffic(kernel, L, R):
-    """HBM bytes per launch measured by counters for this kernel and lattice,
-    from profiles/traffic_*.txt (key=value lines), or None."""
+def read_traffic(kernel, L, R, tiling):
+    """HBM bytes per launch measured by counters for this kernel, lattice and
+    tiling (mode, rows, planes), from profiles/traffic_*.txt (key=value
+    lines), or None."""
     d = os.path.join(ROOT, "profiles")
     if not os.path.isdir(d):
         return None
@@ -71,7 +72,8 @@ def read_traffic(kernel, L, R):
                 if "=" in ln and not ln.startswith("#"):
                     k, v = ln.split("=", 1)
                     kv[k.strip()] = v.strip()
-        if kv.get("kernel") == kernel and kv.get("L") == str(L) and kv.get("R") == str(R):
+        if (kv.get("kernel") == kernel and kv.get("L") == str(L) and kv.get("R") == str(R)
+                and kv.get("tiling") == "%d,%d,%d" % tuple(tiling)):
             return int(kv["hbm_bytes_per_launch"])
     return None
 
@@ -387,7 +389,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": read_traffic(kname, L, R),
+            "traffic": read_traffic(kname, L, R, tiling),
             "kernel": kname,
             "avg_launch_us": round(avg_s * 1e6, 3),
             "launches_timed": launches_k,
