@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
 #include <vector>
 
 #include "msa_common.hpp"
@@ -209,6 +210,13 @@ struct msa_batch {
   BSeg *d_segs = nullptr;                // the blocks' runs of tiles
   uint32_t *d_seg_start = nullptr;      // [nblocks + 1]
   uint32_t nblocks = 0, segs_cap = 0, starts_cap = 0;
+  uint64_t ntiles = 0;                  // tiles of the unfinished lattices in the split
+  // pinned staging of the descriptor, progress and split uploads; ev_up marks
+  // the end of the last upload (the staging is not rewritten before it)
+  char *stage = nullptr;
+  size_t stage_cap = 0;
+  hipEvent_t ev_up = nullptr;
+  bool graph_stale = true;              // launch arguments changed since the capture
   msa::FineLut *luts = nullptr;         // shared search tables, 2 per pair
   int32_t nlut = 0;
   int block = 256;
@@ -256,9 +264,11 @@ extern "C" int msa_batch_create(int device, int64_t max_bytes, msa_batch **out) 
   if (hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || b->ncu < 1)
     b->ncu = 256;
   b->max_bytes = max_bytes > 0 ? max_bytes : INT64_MAX;
-  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_up, hipEventDisableTiming) != hipSuccess) {
+    if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
-    return msa_fail(MSA_ERR_HIP, "stream creation failed");
+    return msa_fail(MSA_ERR_HIP, "stream/event creation failed");
   }
   *out = b;
   return MSA_OK;
@@ -282,6 +292,8 @@ extern "C" int msa_batch_destroy(msa_batch *b) {
   (void)hipFree(b->d_segs);
   (void)hipFree(b->d_seg_start);
   (void)hipFree(b->luts);
+  (void)hipHostFree(b->stage);
+  if (b->ev_up) (void)hipEventDestroy(b->ev_up);
   (void)hipStreamDestroy(b->stream);
   delete b;
   return MSA_OK;
@@ -334,15 +346,16 @@ extern "C" int msa_batch_add(msa_batch *b, const msa_job *j, int32_t *slot_out) 
     return msa_fail(MSA_ERR_NOMEM, "admission: %lld bytes in use + %lld > limit %lld",
                     (long long)b->used_bytes, (long long)bytes, (long long)b->max_bytes);
   // an idle lattice of the same geometry, else make room among the idle ones;
-  // nothing of the batch may still run when an idle lattice's buffers are rewritten
+  // nothing of the batch may still run when an idle lattice's buffers are
+  // rewritten or freed (launches enqueued before its removal may still read it)
   HIPCHK(hipSetDevice(b->device));
-  HIPCHK(hipStreamSynchronize(b->stream));
   msa_ctx *c = nullptr;
   int32_t *d_idx = nullptr;
   int32_t idx_cap = 0;
   for (size_t k = 0; k < b->idle.size(); ++k) {
     const msa_geom &g = b->idle[k].c->geom;
     if (g.L == j->geom.L && g.R == j->geom.R && g.nc == j->geom.nc) {
+      HIPCHK(hipStreamSynchronize(b->stream));
       c = b->idle[k].c;
       d_idx = b->idle[k].idx;
       idx_cap = b->idle[k].cap;
@@ -352,6 +365,7 @@ extern "C" int msa_batch_add(msa_batch *b, const msa_job *j, int32_t *slot_out) 
     }
   }
   while (!c && !b->idle.empty() && b->used_bytes + b->idle_bytes + bytes > b->max_bytes) {
+    HIPCHK(hipStreamSynchronize(b->stream));
     msa_destroy(b->idle.back().c);
     (void)hipFree(b->idle.back().idx);
     b->idle_bytes -= b->idle.back().bytes;
@@ -377,11 +391,10 @@ extern "C" int msa_batch_add(msa_batch *b, const msa_job *j, int32_t *slot_out) 
       idx_cap = j->nref;
     }
     if (hipMemcpyAsync(d_idx, j->lut_idx, sizeof(int32_t) * (size_t)j->nref, hipMemcpyHostToDevice,
-                       b->stream) != hipSuccess ||
-        hipStreamSynchronize(b->stream) != hipSuccess)
+                       c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
       return drop(msa_fail(MSA_ERR_HIP, "refresh index upload"));
   }
-  if ((rc = batch_pull(b))) return drop(rc);
   size_t s = 0;
   while (s < b->slot.size() && b->slot[s]) ++s;
   if (s == b->slot.size()) {
@@ -507,12 +520,34 @@ static void batch_split(const std::vector<uint32_t> &tiles, const std::vector<ui
   start[nblocks] = (uint32_t)segs.size();
 }
 
-// Push descriptors and the work split; (re)capture the graph.
+// Device arrays of at least n elements; a reallocation invalidates the
+// captured graph (its launches hold the old addresses) and waits for the
+// launches already enqueued, which may still read the old array.
+template <class T>
+static int batch_grow(msa_batch *b, T **arr, uint32_t *cap, size_t n, const char *what) {
+  if (n <= *cap) return MSA_OK;
+  const size_t want = std::max<size_t>(n, 2 * (size_t)*cap);
+  HIPCHK(hipStreamSynchronize(b->stream));
+  (void)hipFree(*arr);
+  *arr = nullptr;
+  *cap = 0;
+  if (hipMalloc(arr, sizeof(T) * want) != hipSuccess) return msa_fail(MSA_ERR_NOMEM, "%s", what);
+  *cap = (uint32_t)want;
+  b->graph_stale = true;
+  return MSA_OK;
+}
+
+// Push descriptors, progress and the work split, and capture the graph when
+// its launch arguments changed.  Nothing waits for the device: the uploads
+// go on the batch stream, after the launches already enqueued, and the
+// progress pushed for a running lattice is the host's count of the
+// iterations enqueued for it, which those launches reach exactly.  The grid
+// is the number of blocks the CUs hold at once whatever the batch holds
+// (blocks without a run exit), so admissions and retirements reuse the
+// captured graph.
 static int batch_prepare(msa_batch *b) {
   if (!b->dirty) return MSA_OK;
   const size_t n = b->slot.size();
-  int rc = batch_pull(b);               // keep the progress of lattices already running
-  if (rc) return rc;
   std::vector<uint32_t> tiles, lat;     // unfinished lattices: tile counts and slots
   uint64_t total = 0;
   for (size_t s = 0; s < n; ++s) {
@@ -531,63 +566,68 @@ static int batch_prepare(msa_batch *b) {
       total += L.ntiles;
     }
   }
-  // persistent blocks: as many as the CUs hold at once, fewer when there are
-  // fewer tiles
-  const uint64_t resident = (uint64_t)b->ncu * (uint64_t)batch_resident(b->block);
-  const uint32_t nblocks = (uint32_t)std::min<uint64_t>(resident, total);
+  const uint32_t nblocks = (uint32_t)((uint64_t)b->ncu * (uint64_t)batch_resident(b->block));
+  if (nblocks != b->nblocks) b->graph_stale = true;
   std::vector<BSeg> segs;
   std::vector<uint32_t> start;
-  if (nblocks) batch_split(tiles, lat, nblocks, segs, start);
-  if (n > b->cap) {
+  batch_split(tiles, lat, nblocks, segs, start);
+  if (n > b->cap) {                     // descriptors and progress share one capacity
+    const size_t want = std::max<size_t>(n, 2 * b->cap);
+    HIPCHK(hipStreamSynchronize(b->stream));
     (void)hipFree(b->d_lats);
     (void)hipFree(b->d_prog);
     b->d_lats = nullptr;
     b->d_prog = nullptr;
     b->cap = 0;
-    if (hipMalloc(&b->d_lats, sizeof(BLat) * n) != hipSuccess ||
-        hipMalloc(&b->d_prog, sizeof(BProg) * n) != hipSuccess)
+    if (hipMalloc(&b->d_lats, sizeof(BLat) * want) != hipSuccess ||
+        hipMalloc(&b->d_prog, sizeof(BProg) * want) != hipSuccess)
       return msa_fail(MSA_ERR_NOMEM, "descriptors");
-    b->cap = n;
+    b->cap = want;
+    b->graph_stale = true;
   }
-  std::vector<BProg> prog(n);
+  int rc = MSA_OK;
+  if ((rc = batch_grow(b, &b->d_segs, &b->segs_cap, std::max<size_t>(segs.size(), 1), "work split")) ||
+      (rc = batch_grow(b, &b->d_seg_start, &b->starts_cap, start.size(), "work split")))
+    return rc;
+  // stage everything in pinned memory once the previous upload has left it
+  const size_t o_prog = sizeof(BLat) * n, o_segs = o_prog + sizeof(BProg) * n;
+  const size_t o_start = o_segs + sizeof(BSeg) * segs.size(), bytes = o_start + sizeof(uint32_t) * start.size();
+  HIPCHK(hipEventSynchronize(b->ev_up));
+  if (bytes > b->stage_cap) {
+    (void)hipHostFree(b->stage);
+    b->stage = nullptr;
+    b->stage_cap = 0;
+    if (hipHostMalloc(reinterpret_cast<void **>(&b->stage), 2 * bytes) != hipSuccess)
+      return msa_fail(MSA_ERR_NOMEM, "upload staging");
+    b->stage_cap = 2 * bytes;
+  }
+  BProg *prog = reinterpret_cast<BProg *>(b->stage + o_prog);
   for (size_t s = 0; s < n; ++s) prog[s] = BProg{b->h_lats[s].t, 0, 0};
+  if (n) std::memcpy(b->stage, b->h_lats.data(), sizeof(BLat) * n);
+  if (!segs.empty()) std::memcpy(b->stage + o_segs, segs.data(), sizeof(BSeg) * segs.size());
+  std::memcpy(b->stage + o_start, start.data(), sizeof(uint32_t) * start.size());
   if (n) {
-    HIPCHK(hipMemcpyAsync(b->d_lats, b->h_lats.data(), sizeof(BLat) * n, hipMemcpyHostToDevice, b->stream));
-    HIPCHK(hipMemcpyAsync(b->d_prog, prog.data(), sizeof(BProg) * n, hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipMemcpyAsync(b->d_lats, b->stage, sizeof(BLat) * n, hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipMemcpyAsync(b->d_prog, prog, sizeof(BProg) * n, hipMemcpyHostToDevice, b->stream));
   }
-  for (size_t s = 0; s < n; ++s) b->fresh[s] = 0;
-  if (segs.size() > b->segs_cap) {
-    (void)hipFree(b->d_segs);
-    b->d_segs = nullptr;
-    b->segs_cap = 0;
-    if (hipMalloc(&b->d_segs, sizeof(BSeg) * segs.size()) != hipSuccess)
-      return msa_fail(MSA_ERR_NOMEM, "work split");
-    b->segs_cap = (uint32_t)segs.size();
-  }
-  if (start.size() > b->starts_cap) {
-    (void)hipFree(b->d_seg_start);
-    b->d_seg_start = nullptr;
-    b->starts_cap = 0;
-    if (hipMalloc(&b->d_seg_start, sizeof(uint32_t) * start.size()) != hipSuccess)
-      return msa_fail(MSA_ERR_NOMEM, "work split");
-    b->starts_cap = (uint32_t)start.size();
-  }
-  if (!segs.empty()) {
-    HIPCHK(hipMemcpyAsync(b->d_segs, segs.data(), sizeof(BSeg) * segs.size(), hipMemcpyHostToDevice, b->stream));
-    HIPCHK(hipMemcpyAsync(b->d_seg_start, start.data(), sizeof(uint32_t) * start.size(), hipMemcpyHostToDevice,
+  if (!segs.empty())
+    HIPCHK(hipMemcpyAsync(b->d_segs, b->stage + o_segs, sizeof(BSeg) * segs.size(), hipMemcpyHostToDevice,
                           b->stream));
-  }
-  HIPCHK(hipStreamSynchronize(b->stream));
+  HIPCHK(hipMemcpyAsync(b->d_seg_start, b->stage + o_start, sizeof(uint32_t) * start.size(),
+                        hipMemcpyHostToDevice, b->stream));
+  HIPCHK(hipEventRecord(b->ev_up, b->stream));
+  for (size_t s = 0; s < n; ++s) b->fresh[s] = 0;
   b->nblocks = nblocks;
-  batch_clear_graph(b);
-  if (b->nblocks) {
+  b->ntiles = total;
+  if (b->graph_stale || !b->gexec) {
+    HIPCHK(hipStreamSynchronize(b->stream));   // the old graph may still run
+    batch_clear_graph(b);
     HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     for (int32_t it = 0; it < b->chunk; ++it)
-      for (int col = 0; col < 2; ++col) {
-        batch_launch(b, col);
-      }
+      for (int col = 0; col < 2; ++col) batch_launch(b, col);
     HIPCHK(hipStreamEndCapture(b->stream, &b->graph));
     HIPCHK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+    b->graph_stale = false;
   }
   b->dirty = false;
   return MSA_OK;
@@ -606,6 +646,7 @@ extern "C" int msa_batch_set_shape(msa_batch *b, int32_t block, int32_t chunk) {
   }
   b->chunk = chunk;
   b->dirty = true;
+  b->graph_stale = true;
   return MSA_OK;
 }
 
@@ -620,11 +661,10 @@ extern "C" int msa_batch_run(msa_batch *b, int64_t niter) {
   int64_t left = niter;
   while (left > 0) {
     if (b->dirty) {
-      HIPCHK(hipStreamSynchronize(b->stream));
       int rc = batch_prepare(b);
       if (rc) return rc;
     }
-    if (!b->nblocks) break;
+    if (!b->ntiles) break;
     int64_t step = left;
     for (size_t s = 0; s < b->slot.size(); ++s)
       if (b->slot[s] && b->h_lats[s].t < b->h_lats[s].tend)
@@ -700,12 +740,11 @@ extern "C" int msa_batch_remove(msa_batch *b, int32_t slot) {
   if (!b || slot < 0 || (size_t)slot >= b->slot.size() || !b->slot[slot])
     return msa_fail(MSA_ERR_ARG, "no lattice in slot %d", slot);
   HIPCHK(hipSetDevice(b->device));
-  int rc = batch_pull(b);               // keep the progress of the others
-  if (rc) return rc;
   if (b->idle.size() < 32) {            // keep it for a later admission of its geometry
     b->idle.push_back({b->slot[slot], b->slot_lut_idx[slot], b->slot_idx_cap[slot], b->slot_bytes[slot]});
     b->idle_bytes += b->slot_bytes[slot];
   } else {
+    HIPCHK(hipStreamSynchronize(b->stream));   // enqueued launches may still read it
     msa_destroy(b->slot[slot]);
     (void)hipFree(b->slot_lut_idx[slot]);
   }

@@ -11,6 +11,7 @@ because each lattice's update depends only on its own state and tables.
 
 No reference counterpart: the reference runs one sample set per process.
 """
+import concurrent.futures
 import dataclasses
 import time
 
@@ -107,22 +108,27 @@ def run_alone(req, tables, jobid):
     return out
 
 
-def serve(requests, max_bytes, epoch=64, block=None, tables=None, keep_state=False, clock=None):
+def serve(requests, max_bytes, epoch=64, block=None, tables=None, keep_state=False, clock=None,
+          workers=4):
     """Run the stream; returns (per-request dicts, summary dict).  block=None
     picks the batch kernel's block size with tune_block on one request per
-    lattice size of the stream."""
+    lattice size of the stream.  The refresh tables of arrived requests are
+    built by `workers` host threads (the host library releases the
+    interpreter lock) while the device runs; a request is admitted, in
+    arrival order, once its tables are ready and its footprint fits."""
     tables = tables or Tables()
     if block is None:
         block, _ = tune_block(tables, size_sample(sorted({r.L for r in requests})))
     batch = msa.Batch(tables.luts, max_bytes=max_bytes)
     batch.set_shape(block, epoch)
+    pool = concurrent.futures.ThreadPoolExecutor(max_workers=workers)
     reqs = sorted(requests, key=lambda r: r.arrival)
     clock = clock or time.perf_counter
     t_start = clock()
     nxt = 0
     queue = []
     active = {}                 # slot -> (request, admit time)
-    prepared = {}               # rid -> job tables built while the device ran
+    prepared = {}               # rid -> future of its job tables
     done = {}
     updates = 0
     max_active = 0
@@ -130,18 +136,22 @@ def serve(requests, max_bytes, epoch=64, block=None, tables=None, keep_state=Fal
     while len(done) < len(reqs):
         now = clock() - t_start
         while nxt < len(reqs) and reqs[nxt].arrival <= now:
-            queue.append(reqs[nxt])
+            req = reqs[nxt]
+            prepared[req.rid] = pool.submit(tables.job, req, req.rid)
+            queue.append(req)
             nxt += 1
         while queue:                              # FIFO admission by footprint
             req = queue[0]
-            job = prepared.pop(req.rid, None) or tables.job(req, req.rid)
-            slot = batch.add(job)
+            fut = prepared[req.rid]
+            if not fut.done() and active:
+                break                             # admitted at a later epoch
+            slot = batch.add(fut.result())
             if slot is None:
                 if not active:
                     raise RuntimeError("request %d alone exceeds the memory limit" % req.rid)
-                prepared[req.rid] = job
                 break
             queue.pop(0)
+            del prepared[req.rid]
             active[slot] = (req, clock() - t_start)
         max_active = max(max_active, len(active))
         max_bytes_used = max(max_bytes_used, batch.bytes_in_use())
@@ -150,10 +160,6 @@ def serve(requests, max_bytes, epoch=64, block=None, tables=None, keep_state=Fal
                 time.sleep(max(0.0, min(0.01, reqs[nxt].arrival - (clock() - t_start))))
             continue
         batch.run(epoch)
-        # while the device runs the epoch: the tables of queued requests
-        for req in queue:
-            if req.rid not in prepared:
-                prepared[req.rid] = tables.job(req, req.rid)
         nslots = max(active) + 1
         t, tend = batch.progress(nslots)
         for slot in [s for s in active if t[s] >= tend[s]]:
@@ -168,6 +174,7 @@ def serve(requests, max_bytes, epoch=64, block=None, tables=None, keep_state=Fal
             updates += 64 * req.R * req.L ** 3 * req.niter
             batch.remove(slot)
     makespan = clock() - t_start
+    pool.shutdown()
     batch.close()
     lat = np.array([done[r.rid]["latency"] for r in reqs])
     summary = {"requests": len(reqs), "makespan_s": makespan,
