@@ -219,15 +219,26 @@ int msa_batch_upload_state(msa_batch *b, int32_t slot, const uint64_t *J, const 
                            const uint64_t *d0, const uint64_t *d1);
 /* Launch shape: threads per block (128, 256, 512, 1024) and iterations per graph. */
 int msa_batch_set_shape(msa_batch *b, int32_t block, int32_t chunk);
-/* Advance every unfinished lattice by up to niter iterations (asynchronous). */
+/* Advance every unfinished lattice by up to niter iterations (asynchronous:
+ * the launches are enqueued on the batch's stream). */
 int msa_batch_run(msa_batch *b, int64_t niter);
 int msa_batch_sync(msa_batch *b);
-/* Iteration reached and length of slots 0..nslots-1 (-1 for empty slots). */
+/* Wait for the work enqueued before the latest msa_batch_run, so a caller can
+ * keep one run queued on the device while it handles the previous one. */
+int msa_batch_wait_previous(msa_batch *b);
+/* Iteration reached and length of slots 0..nslots-1 (-1 for empty slots),
+ * read from the device after a sync. */
 int msa_batch_progress(msa_batch *b, int32_t nslots, int64_t *t, int64_t *tend);
+/* The same counts as planned by the host: the iterations enqueued so far,
+ * which the device reaches when it finishes that work (no sync). */
+int msa_batch_planned(msa_batch *b, int32_t nslots, int64_t *t, int64_t *tend);
+/* Per-lane observables and state of a slot.  For a lattice that has reached
+ * its length these wait only for the launches that finished it (on the
+ * lattice's own stream), not for later batch work; otherwise for the batch. */
 int msa_batch_measure(msa_batch *b, int32_t slot, int32_t *unsat, int32_t *demon, int32_t *q,
                       int32_t *qlink);
 int msa_batch_download_state(msa_batch *b, int32_t slot, uint64_t *spin, uint64_t *d0, uint64_t *d1);
-/* Retire a lattice.  Its device buffers are kept (up to 32 lattices) and
+/* Retire a lattice.  Its device buffers are kept (up to 256 lattices) and
  * reused by a later admission of the same geometry, so a serving loop does no
  * device allocation in steady state; lattices kept this way count against
  * max_bytes together with those admitted, and are freed first when a new

@@ -199,6 +199,7 @@ struct msa_batch {
     int32_t *idx;
     int32_t cap;
     int64_t bytes;
+    bool quiet;   // finished and its last launches completed: no launch reads its buffers
   };
   std::vector<Idle> idle;
   int64_t idle_bytes = 0;
@@ -217,6 +218,12 @@ struct msa_batch {
   size_t stage_cap = 0;
   hipEvent_t ev_up = nullptr;
   bool graph_stale = true;              // launch arguments changed since the capture
+  // per slot: an event recorded after the launches in which the lattice
+  // reached its length (ev_set), so its results can be read on its own
+  // stream without waiting for later batch work
+  std::vector<hipEvent_t> slot_ev;
+  std::vector<char> slot_ev_set;
+  hipEvent_t ev_prev = nullptr;         // end of the work enqueued before the last msa_batch_run
   msa::FineLut *luts = nullptr;         // shared search tables, 2 per pair
   int32_t nlut = 0;
   int block = 256;
@@ -265,7 +272,8 @@ extern "C" int msa_batch_create(int device, int64_t max_bytes, msa_batch **out) 
     b->ncu = 256;
   b->max_bytes = max_bytes > 0 ? max_bytes : INT64_MAX;
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreateWithFlags(&b->ev_up, hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&b->ev_up, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_prev, hipEventDisableTiming) != hipSuccess) {
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return msa_fail(MSA_ERR_HIP, "stream/event creation failed");
@@ -294,6 +302,9 @@ extern "C" int msa_batch_destroy(msa_batch *b) {
   (void)hipFree(b->luts);
   (void)hipHostFree(b->stage);
   if (b->ev_up) (void)hipEventDestroy(b->ev_up);
+  if (b->ev_prev) (void)hipEventDestroy(b->ev_prev);
+  for (hipEvent_t e : b->slot_ev)
+    if (e) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(b->stream);
   delete b;
   return MSA_OK;
@@ -355,7 +366,7 @@ extern "C" int msa_batch_add(msa_batch *b, const msa_job *j, int32_t *slot_out) 
   for (size_t k = 0; k < b->idle.size(); ++k) {
     const msa_geom &g = b->idle[k].c->geom;
     if (g.L == j->geom.L && g.R == j->geom.R && g.nc == j->geom.nc) {
-      HIPCHK(hipStreamSynchronize(b->stream));
+      if (!b->idle[k].quiet) HIPCHK(hipStreamSynchronize(b->stream));
       c = b->idle[k].c;
       d_idx = b->idle[k].idx;
       idx_cap = b->idle[k].cap;
@@ -404,7 +415,10 @@ extern "C" int msa_batch_add(msa_batch *b, const msa_job *j, int32_t *slot_out) 
     b->slot_idx_cap.push_back(0);
     b->h_lats.push_back(BLat{});
     b->fresh.push_back(0);
+    b->slot_ev.push_back(nullptr);
+    b->slot_ev_set.push_back(0);
   }
+  b->slot_ev_set[s] = 0;
   b->fresh[s] = 1;
   b->slot[s] = c;
   b->slot_bytes[s] = bytes;
@@ -654,6 +668,7 @@ extern "C" int msa_batch_run(msa_batch *b, int64_t niter) {
   if (!b) return msa_fail(MSA_ERR_ARG, "NULL batch");
   if (niter < 0) return msa_fail(MSA_ERR_ARG, "niter=%lld", (long long)niter);
   HIPCHK(hipSetDevice(b->device));
+  HIPCHK(hipEventRecord(b->ev_prev, b->stream));
   // Every unfinished lattice advances one iteration per iteration, so the
   // host knows where each reaches its length: the run is cut there and the
   // work list rebuilt without the finished lattices (their blocks would only
@@ -681,7 +696,13 @@ extern "C" int msa_batch_run(msa_batch *b, int64_t niter) {
       BLat &L = b->h_lats[s];
       if (!b->slot[s] || L.t >= L.tend) continue;
       L.t += (uint64_t)step;
-      if (L.t >= L.tend) b->dirty = true;
+      if (L.t >= L.tend) {
+        b->dirty = true;
+        if (!b->slot_ev[s] && hipEventCreateWithFlags(&b->slot_ev[s], hipEventDisableTiming) != hipSuccess)
+          return msa_fail(MSA_ERR_HIP, "event creation failed");
+        HIPCHK(hipEventRecord(b->slot_ev[s], b->stream));
+        b->slot_ev_set[s] = 1;
+      }
     }
     left -= step;
   }
@@ -692,6 +713,23 @@ extern "C" int msa_batch_sync(msa_batch *b) {
   if (!b) return msa_fail(MSA_ERR_ARG, "NULL batch");
   HIPCHK(hipSetDevice(b->device));
   HIPCHK(hipStreamSynchronize(b->stream));
+  return MSA_OK;
+}
+
+extern "C" int msa_batch_wait_previous(msa_batch *b) {
+  if (!b) return msa_fail(MSA_ERR_ARG, "NULL batch");
+  HIPCHK(hipSetDevice(b->device));
+  HIPCHK(hipEventSynchronize(b->ev_prev));
+  return MSA_OK;
+}
+
+extern "C" int msa_batch_planned(msa_batch *b, int32_t nslots, int64_t *t, int64_t *tend) {
+  if (!b || nslots < 0) return msa_fail(MSA_ERR_ARG, "bad argument");
+  for (int32_t s = 0; s < nslots; ++s) {
+    const bool live = (size_t)s < b->slot.size() && b->slot[s];
+    if (t) t[s] = live ? (int64_t)b->h_lats[s].t : -1;
+    if (tend) tend[s] = live ? (int64_t)b->h_lats[s].tend : -1;
+  }
   return MSA_OK;
 }
 
@@ -713,10 +751,16 @@ static int batch_ctx(msa_batch *b, int32_t slot, msa_ctx **out) {
   if (!b || slot < 0 || (size_t)slot >= b->slot.size() || !b->slot[slot])
     return msa_fail(MSA_ERR_ARG, "no lattice in slot %d", slot);
   HIPCHK(hipSetDevice(b->device));
-  int rc = batch_pull(b);
-  if (rc) return rc;
-  const uint64_t t = b->h_lats[slot].t;
   msa_ctx *c = b->slot[slot];
+  if (b->slot_ev_set[slot]) {
+    // finished: its state is final once the launches that finished it are
+    // done, and later batch work does not touch it
+    HIPCHK(hipStreamWaitEvent(c->stream, b->slot_ev[slot], 0));
+  } else {
+    int rc = batch_pull(b);
+    if (rc) return rc;
+  }
+  const uint64_t t = b->h_lats[slot].t;
   c->scur = c->cur = (int)(t & 1u);
   *out = c;
   return MSA_OK;
@@ -740,8 +784,10 @@ extern "C" int msa_batch_remove(msa_batch *b, int32_t slot) {
   if (!b || slot < 0 || (size_t)slot >= b->slot.size() || !b->slot[slot])
     return msa_fail(MSA_ERR_ARG, "no lattice in slot %d", slot);
   HIPCHK(hipSetDevice(b->device));
-  if (b->idle.size() < 32) {            // keep it for a later admission of its geometry
-    b->idle.push_back({b->slot[slot], b->slot_lut_idx[slot], b->slot_idx_cap[slot], b->slot_bytes[slot]});
+  if (b->idle.size() < 256) {           // keep it for a later admission of its geometry
+    // quiet when it finished and the launches that finished it completed
+    const bool quiet = b->slot_ev_set[slot] && hipEventQuery(b->slot_ev[slot]) == hipSuccess;
+    b->idle.push_back({b->slot[slot], b->slot_lut_idx[slot], b->slot_idx_cap[slot], b->slot_bytes[slot], quiet});
     b->idle_bytes += b->slot_bytes[slot];
   } else {
     HIPCHK(hipStreamSynchronize(b->stream));   // enqueued launches may still read it
@@ -754,6 +800,7 @@ extern "C" int msa_batch_remove(msa_batch *b, int32_t slot) {
   b->slot_bytes[slot] = 0;
   b->h_lats[slot] = BLat{};
   b->fresh[slot] = 0;
+  b->slot_ev_set[slot] = 0;
   b->dirty = true;
   return MSA_OK;
 }

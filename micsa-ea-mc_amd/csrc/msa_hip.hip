@@ -828,10 +828,18 @@ static int check_geom(const msa_geom *g) {
   return MSA_OK;
 }
 
+// Bytes of a context's state allocation: spins and demons twice (ping-pong),
+// J, the measurement counts, each rounded up to 256 bytes.
+static size_t state_bytes(const msa_geom *geom) {
+  const size_t V = (size_t)geom->L * geom->L * geom->L;
+  auto up = [](size_t n) { return (n + 255) & ~(size_t)255; };
+  return 2 * up(8 * V * geom->R) + up(8 * V * 3) + 2 * up(16 * V * geom->R * geom->nc) +
+         up(64 * sizeof(int32_t));
+}
+
 extern "C" int64_t msa_footprint_bytes(const msa_geom *geom) {
   if (check_geom(geom) != MSA_OK) return -1;
-  const int64_t V = (int64_t)geom->L * geom->L * geom->L;
-  return 8 * V * (3 + 2 * geom->R) + 2 * 16 * V * geom->R * geom->nc + 256;
+  return (int64_t)state_bytes(geom);
 }
 
 // Tile of the two-pass kernel: nr rows of L/2 site pairs per block.  Blocks
@@ -940,14 +948,22 @@ extern "C" int msa_create(int device, const msa_geom *geom, msa_ctx **out) {
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
     return cleanup(msa_fail(MSA_ERR_HIP, "stream/event creation failed"));
-  if (hipMalloc(&c->sp[0], 8 * V * geom->R) != hipSuccess ||
-      hipMalloc(&c->sp[1], 8 * V * geom->R) != hipSuccess ||
-      hipMalloc(&c->jp, 8 * V * 3) != hipSuccess ||
-      hipMalloc(&c->dm[0], 16 * V * geom->R * geom->nc) != hipSuccess ||
-      hipMalloc(&c->dm[1], 16 * V * geom->R * geom->nc) != hipSuccess ||
-      hipMalloc(&c->meas, 64 * sizeof(int32_t)) != hipSuccess)
+  // one allocation for the state (spins and demons ping-pong, J, the
+  // measurement counts), each array at a 256-byte boundary
+  const size_t nsp = 8 * V * geom->R, njp = 8 * V * 3, ndm = 16 * V * geom->R * geom->nc;
+  auto up = [](size_t n) { return (n + 255) & ~(size_t)255; };
+  if (hipMalloc(&c->slab, state_bytes(geom)) != hipSuccess)
     return cleanup(msa_fail(MSA_ERR_NOMEM, "device allocation of %lld bytes failed",
                         (long long)msa_footprint_bytes(geom)));
+  {
+    char *q = static_cast<char *>(c->slab);
+    c->sp[0] = reinterpret_cast<uint64_t *>(q); q += up(nsp);
+    c->sp[1] = reinterpret_cast<uint64_t *>(q); q += up(nsp);
+    c->jp = reinterpret_cast<uint64_t *>(q); q += up(njp);
+    c->dm[0] = reinterpret_cast<ulonglong2 *>(q); q += up(ndm);
+    c->dm[1] = reinterpret_cast<ulonglong2 *>(q); q += up(ndm);
+    c->meas = reinterpret_cast<int32_t *>(q);
+  }
   // every transfer and clear of a context goes on its own (non-blocking)
   // stream: the legacy null stream does not order against it
   if (hipMemsetAsync(c->dm[0], 0, 16 * V * geom->R * geom->nc, c->stream) != hipSuccess ||
@@ -973,15 +989,10 @@ extern "C" int msa_destroy(msa_ctx *c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   walkers_free(c);
-  (void)hipFree(c->sp[0]);
-  (void)hipFree(c->sp[1]);
-  (void)hipFree(c->jp);
-  (void)hipFree(c->dm[0]);
-  (void)hipFree(c->dm[1]);
+  (void)hipFree(c->slab);
   if (c->luts_owned) (void)hipFree(c->luts);
   (void)hipFree(c->ctr);
   (void)hipFree(c->key);
-  (void)hipFree(c->meas);
   (void)hipFree(c->th_rnd);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);

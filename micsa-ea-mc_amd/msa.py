@@ -87,6 +87,8 @@ def hip():
         L.msa_batch_run.argtypes = [bp, C.c_int64]
         L.msa_batch_sync.argtypes = [bp]
         L.msa_batch_progress.argtypes = [bp, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        L.msa_batch_planned.argtypes = [bp, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        L.msa_batch_wait_previous.argtypes = [bp]
         L.msa_batch_measure.argtypes = [bp, C.c_int32, i32p, i32p, i32p, i32p]
         L.msa_batch_download_state.argtypes = [bp, C.c_int32, u64p, u64p, u64p]
         L.msa_batch_remove.argtypes = [bp, C.c_int32]
@@ -536,11 +538,24 @@ class Batch:
         _check(self.lib.msa_batch_sync(self.b))
 
     def progress(self, nslots):
+        """Iterations reached and lengths, from the device (syncs)."""
         t = np.zeros(nslots, np.int64)
         te = np.zeros(nslots, np.int64)
         _check(self.lib.msa_batch_progress(self.b, nslots, t.ctypes.data_as(C.POINTER(C.c_int64)),
                                            te.ctypes.data_as(C.POINTER(C.c_int64))))
         return t, te
+
+    def planned(self, nslots):
+        """Iterations enqueued so far and lengths (no sync)."""
+        t = np.zeros(nslots, np.int64)
+        te = np.zeros(nslots, np.int64)
+        _check(self.lib.msa_batch_planned(self.b, nslots, t.ctypes.data_as(C.POINTER(C.c_int64)),
+                                          te.ctypes.data_as(C.POINTER(C.c_int64))))
+        return t, te
+
+    def wait_previous(self):
+        """Wait for the work enqueued before the latest run."""
+        _check(self.lib.msa_batch_wait_previous(self.b))
 
     def download(self, slot):
         L, R = self.geom[slot]

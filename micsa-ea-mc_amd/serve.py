@@ -133,6 +133,21 @@ def serve(requests, max_bytes, epoch=64, block=None, tables=None, keep_state=Fal
     updates = 0
     max_active = 0
     max_bytes_used = 0
+    finishing = []              # slots that reach their length in the run enqueued last
+
+    def retire(slot):
+        nonlocal updates
+        req, t_admit = active.pop(slot)
+        fin = clock() - t_start
+        rec = {"rid": req.rid, "L": req.L, "niter": req.niter, "arrival": req.arrival,
+               "admit": t_admit, "finish": fin, "latency": fin - req.arrival,
+               "energy": batch.measure(slot)["energy"]}
+        if keep_state:
+            rec["state"] = batch.download(slot)
+        done[req.rid] = rec
+        updates += 64 * req.R * req.L ** 3 * req.niter
+        batch.remove(slot)
+
     while len(done) < len(reqs):
         now = clock() - t_start
         while nxt < len(reqs) and reqs[nxt].arrival <= now:
@@ -155,24 +170,24 @@ def serve(requests, max_bytes, epoch=64, block=None, tables=None, keep_state=Fal
             active[slot] = (req, clock() - t_start)
         max_active = max(max_active, len(active))
         max_bytes_used = max(max_bytes_used, batch.bytes_in_use())
-        if not active:
+        running = [s for s in active if s not in finishing]
+        if not running and not finishing:
             if nxt < len(reqs):
                 time.sleep(max(0.0, min(0.01, reqs[nxt].arrival - (clock() - t_start))))
             continue
-        batch.run(epoch)
-        nslots = max(active) + 1
-        t, tend = batch.progress(nslots)
-        for slot in [s for s in active if t[s] >= tend[s]]:
-            req, t_admit = active.pop(slot)
-            fin = clock() - t_start
-            rec = {"rid": req.rid, "L": req.L, "niter": req.niter, "arrival": req.arrival,
-                   "admit": t_admit, "finish": fin, "latency": fin - req.arrival,
-                   "energy": batch.measure(slot)["energy"]}
-            if keep_state:
-                rec["state"] = batch.download(slot)
-            done[req.rid] = rec
-            updates += 64 * req.R * req.L ** 3 * req.niter
-            batch.remove(slot)
+        # one epoch stays queued on the device while the previous one's
+        # finished requests are measured and retired
+        new_finishing = []
+        if running:
+            batch.run(epoch)
+            t, tend = batch.planned(max(running) + 1)
+            new_finishing = [s for s in running if t[s] >= tend[s]]
+            batch.wait_previous()
+        else:
+            batch.sync()
+        for slot in finishing:
+            retire(slot)
+        finishing = new_finishing
     makespan = clock() - t_start
     pool.shutdown()
     batch.close()
