@@ -570,10 +570,12 @@ __device__ __forceinline__ void tile_of_pair(uint32_t b, uint32_t nty, uint32_t 
 // below (the thread's own column, in registers).  Only the second sweep's
 // results of the own rows and planes are written.  The first sweep's demons
 // walk (layer 0: colour 0 by +1, colour 1 by -1 in the split index) to the
-// neighbouring thread of the same row through LDS two steps after they are
-// produced.  Every LDS buffer is double-buffered, so one barrier per step
-// orders all writers and readers.  J of the second sweep's planes is read
-// again (the first sweep loaded it two steps earlier; the lines are in L2).
+// neighbouring thread of the same row: they are written to a three-slot LDS
+// ring as they are produced and read by the second sweep two steps later.
+// J is loaded once, by the first sweep: every J word the second sweep needs
+// is one the same thread loaded two to four steps earlier, kept in register
+// windows.  The spin and result planes are double-buffered, so one barrier
+// per step orders all writers and readers.
 __global__ void __launch_bounds__(1024) k_sweep_pair(PairArgs a) {
   using msa::ldo;
   using msa::ldo_stream;
@@ -596,7 +598,7 @@ __global__ void __launch_bounds__(1024) k_sweep_pair(PairArgs a) {
   const bool hrow = row == 0 || row + 1 == ny;
   const uint32_t P = ny * Lh, slot = row * Lh + i;
   uint64_t *S1p = s_lds, *A0b = s_lds + 2 * P, *A1b = s_lds + 4 * P, *B0b = s_lds + 6 * P;
-  ulonglong2 *XD0 = reinterpret_cast<ulonglong2 *>(s_lds + 8 * P), *XD1 = XD0 + 2 * P;
+  ulonglong2 *XD0 = reinterpret_cast<ulonglong2 *>(s_lds + 8 * P), *XD1 = XD0 + 3 * P;
   const uint64_t *S0in = a.sp_in + (uint64_t)(2 * r) * H, *S1in = S0in + H;
   uint64_t *S0out = a.sp_out + (uint64_t)(2 * r) * H, *S1out = S0out + H;
   const ulonglong2 *Din0 = a.dm_in + (uint64_t)(2 * r) * H, *Din1 = Din0 + H;
@@ -614,10 +616,13 @@ __global__ void __launch_bounds__(1024) k_sweep_pair(PairArgs a) {
   uint64_t E0 = 0, Em1 = 0, Em2 = 0;      // A0 results, planes s, s-1, s-2
   uint64_t F0 = 0, F1 = 0, F2 = 0;        // A1 results, planes s-1, s-2, s-3
   uint64_t G0 = 0, G1 = 0, G2 = 0;        // B0 results, planes s-2, s-3, s-4
-  uint64_t a0d0[3] = {0, 0, 0}, a0d1[3] = {0, 0, 0};   // A0 demons, planes s, s-1, s-2
-  uint64_t a1d0[3] = {0, 0, 0}, a1d1[3] = {0, 0, 0};   // A1 demons, planes s-1, s-2, s-3
-  uint64_t j1z_prev = ldo(Jz1, (i + yb + LhL * wrapi((int)z0 - 4, L)) * 8u);
-  uint64_t j1x_prev = 0, j1y_prev = 0, j0z_m1 = 0, j0z_m2 = 0;
+  // J windows, [d] = loaded d+1 steps ago: the first sweep's own bonds
+  // (colour 0 at plane s-1-d: x, y, z; colour 1: x, y, z), the colour-1 bonds
+  // of A0's -x and -y neighbours and the colour-0 bonds of A1's
+  uint64_t w0x[2] = {0, 0}, w0y[2] = {0, 0}, w0z[4] = {0, 0, 0, 0};
+  uint64_t w1x[3] = {0, 0, 0}, w1y[3] = {0, 0, 0}, w1z[3] = {0, 0, 0};
+  uint64_t wc0x[2] = {0, 0}, wc0y[2] = {0, 0}, wc1x[2] = {0, 0}, wc1y[2] = {0, 0};
+  w1z[0] = ldo(Jz1, (i + yb + LhL * wrapi((int)z0 - 4, L)) * 8u);
 
   const int nstep = (int)a.tz + 6;
   for (int k = 0; k < nstep; ++k) {
@@ -628,7 +633,8 @@ __global__ void __launch_bounds__(1024) k_sweep_pair(PairArgs a) {
     const uint32_t off = (y + z) & 1u;
     const uint32_t ip = off ? (i + 1 == Lh ? 0 : i + 1) : i, im = off ? i : (i == 0 ? Lh - 1 : i - 1);
     const int cur = k & 1, prv = cur ^ 1;
-    // ---- global loads of the step and the LDS writes before the barrier
+    const uint32_t ring = (uint32_t)(k % 3), ring2 = (uint32_t)((k + 1) % 3);   // slots of k and k-2
+    // ---- global loads of the step and the LDS write before the barrier
     Sp = ldo(S1in, (i + yb + LhL * wrapi(s + 1, L)) * 8u);
     const uint64_t j0x = ldo(Jx0, h * 8u), j0y = ldo(Jy0, h * 8u), j0z = ldo(Jz0, h * 8u);
     const uint64_t j1x = ldo(Jx1, h * 8u), j1y = ldo(Jy1, h * 8u), j1z = ldo(Jz1, h * 8u);
@@ -637,8 +643,6 @@ __global__ void __launch_bounds__(1024) k_sweep_pair(PairArgs a) {
     const ulonglong2 dA0 = ldo_stream(Din0, h * 16u);
     const uint64_t Hv = hrow ? ldo(S1in, (i + hrb + LhL * z) * 8u) : 0;
     S1p[cur * P + slot] = S0w;
-    XD0[cur * P + row * Lh + iw0] = make_ulonglong2(a0d0[2], a0d1[2]);
-    XD1[cur * P + row * Lh + iw1] = make_ulonglong2(a1d0[2], a1d1[2]);
     __syncthreads();
     // ---- A0: first sweep, colour 0, plane s
     {
@@ -647,37 +651,33 @@ __global__ void __launch_bounds__(1024) k_sweep_pair(PairArgs a) {
       const uint64_t nxp = Sl[row * Lh + ip], nxm = Sl[row * Lh + im];
       const uint64_t nyp = row + 1 == ny ? Hv : Sl[rp * Lh + i];
       const uint64_t nym = row == 0 ? Hv : Sl[rm * Lh + i];
-      E0 = msa::site_update(S0v, nxp, nyp, Sp, nxm, nym, Sm, j0x, j0y, j0z, c0J3, c0J4, j1z_prev,
+      E0 = msa::site_update(S0v, nxp, nyp, Sp, nxm, nym, Sm, j0x, j0y, j0z, c0J3, c0J4, w1z[0],
                             D0, D1);
       A0b[cur * P + slot] = E0;
-      a0d0[0] = D0;
-      a0d1[0] = D1;
+      XD0[ring * P + row * Lh + iw0] = make_ulonglong2(D0, D1);   // walked, read by B0 at k+2
     }
     // ---- A1: first sweep, colour 1, plane s-1
+    uint64_t c1J3 = 0, c1J4 = 0;
     if (rowA1 && k >= 2) {
-      const uint64_t c1J3 = ldo(Jx0, (im + yb + LhL * z1) * 8u);
-      const uint64_t c1J4 = ldo(Jy0, (i + ymb + LhL * z1) * 8u);
+      c1J3 = ldo(Jx0, (im + yb + LhL * z1) * 8u);
+      c1J4 = ldo(Jy0, (i + ymb + LhL * z1) * 8u);
       const ulonglong2 d = ldo_stream(Din1, h1 * 16u);
       uint64_t D0 = d.x, D1 = d.y;
       const uint64_t *El = A0b + prv * P;
       F0 = msa::site_update(Sm, El[row * Lh + ip], El[rp * Lh + i], E0, El[row * Lh + im],
-                            El[rm * Lh + i], Em2, j1x_prev, j1y_prev, j1z_prev, c1J3, c1J4, j0z_m2,
+                            El[rm * Lh + i], Em2, w1x[0], w1y[0], w1z[0], c1J3, c1J4, w0z[1],
                             D0, D1);
       A1b[cur * P + slot] = F0;
-      a1d0[0] = D0;
-      a1d1[0] = D1;
+      XD1[ring * P + row * Lh + iw1] = make_ulonglong2(D0, D1);   // walked, read by B1 at k+2
     }
-    // ---- B0: second sweep, colour 0, plane s-2
+    // ---- B0: second sweep, colour 0, plane s-2 (J of A0 two steps ago)
     if (rowB0 && k >= 4) {
-      const uint64_t bj0 = ldo(Jx0, h2 * 8u), bj1 = ldo(Jy0, h2 * 8u), bj2 = ldo(Jz0, h2 * 8u);
-      const uint64_t bj3 = ldo(Jx1, (im + yb + LhL * z2) * 8u);
-      const uint64_t bj4 = ldo(Jy1, (i + ymb + LhL * z2) * 8u);
-      const uint64_t bj5 = ldo(Jz1, h3 * 8u);
-      const ulonglong2 d = XD0[cur * P + slot];
+      const ulonglong2 d = XD0[ring2 * P + slot];
       uint64_t D0 = d.x, D1 = d.y;
       const uint64_t *Fl = A1b + prv * P;
       G0 = msa::site_update(Em2, Fl[row * Lh + ip], Fl[rp * Lh + i], F0, Fl[row * Lh + im],
-                            Fl[rm * Lh + i], F2, bj0, bj1, bj2, bj3, bj4, bj5, D0, D1);
+                            Fl[rm * Lh + i], F2, w0x[1], w0y[1], w0z[1], wc0x[1], wc0y[1], w1z[2],
+                            D0, D1);
       B0b[cur * P + slot] = G0;
       if (rowB1 && k >= 5 && k <= (int)a.tz + 4) {
         sto_stream(S0out, h2 * 8u, G0);
@@ -686,17 +686,12 @@ __global__ void __launch_bounds__(1024) k_sweep_pair(PairArgs a) {
     }
     // ---- B1: second sweep, colour 1, plane s-3 (the tile's own rows and planes)
     if (rowB1 && k >= 6) {
-      const uint32_t z4 = wrapi(s - 4, L);
-      const uint64_t bj0 = ldo(Jx1, h3 * 8u), bj1 = ldo(Jy1, h3 * 8u), bj2 = ldo(Jz1, h3 * 8u);
-      const uint64_t bj3 = ldo(Jx0, (im + yb + LhL * z3) * 8u);
-      const uint64_t bj4 = ldo(Jy0, (i + ymb + LhL * z3) * 8u);
-      const uint64_t bj5 = ldo(Jz0, (i + yb + LhL * z4) * 8u);
-      const ulonglong2 d = XD1[cur * P + slot];
+      const ulonglong2 d = XD1[ring2 * P + slot];
       uint64_t D0 = d.x, D1 = d.y;
       const uint64_t *Gl = B0b + prv * P;
       const uint64_t sn = msa::site_update(F2, Gl[row * Lh + ip], Gl[rp * Lh + i], G0,
-                                           Gl[row * Lh + im], Gl[rm * Lh + i], G2, bj0, bj1, bj2,
-                                           bj3, bj4, bj5, D0, D1);
+                                           Gl[row * Lh + im], Gl[rm * Lh + i], G2, w1x[2], w1y[2],
+                                           w1z[2], wc1x[1], wc1y[1], w0z[3], D0, D1);
       sto_stream(S1out, h3 * 8u, sn);
       sto_stream(Dout1, (iw1 + yb + LhL * z3) * 16u, D0, D1);
     }
@@ -709,19 +704,16 @@ __global__ void __launch_bounds__(1024) k_sweep_pair(PairArgs a) {
     F1 = F0;
     G2 = G1;
     G1 = G0;
-    a0d0[2] = a0d0[1];
-    a0d1[2] = a0d1[1];
-    a0d0[1] = a0d0[0];
-    a0d1[1] = a0d1[0];
-    a1d0[2] = a1d0[1];
-    a1d1[2] = a1d1[1];
-    a1d0[1] = a1d0[0];
-    a1d1[1] = a1d1[0];
-    j1x_prev = j1x;
-    j1y_prev = j1y;
-    j1z_prev = j1z;
-    j0z_m2 = j0z_m1;
-    j0z_m1 = j0z;
+    w0x[1] = w0x[0]; w0x[0] = j0x;
+    w0y[1] = w0y[0]; w0y[0] = j0y;
+    w0z[3] = w0z[2]; w0z[2] = w0z[1]; w0z[1] = w0z[0]; w0z[0] = j0z;
+    w1x[2] = w1x[1]; w1x[1] = w1x[0]; w1x[0] = j1x;
+    w1y[2] = w1y[1]; w1y[1] = w1y[0]; w1y[0] = j1y;
+    w1z[2] = w1z[1]; w1z[1] = w1z[0]; w1z[0] = j1z;
+    wc0x[1] = wc0x[0]; wc0x[0] = c0J3;
+    wc0y[1] = wc0y[0]; wc0y[0] = c0J4;
+    wc1x[1] = wc1x[0]; wc1x[0] = c1J3;
+    wc1y[1] = wc1y[0]; wc1y[0] = c1J4;
   }
 }
 
@@ -882,9 +874,9 @@ static bool narrow_ok(const Geo &g) {
   return 16ull * g.R * g.nc * g.V < lim && 24ull * g.V < lim && 8ull * g.R * g.V < lim;
 }
 
-// LDS of the sweep-pair kernel: eight planes of 8-byte words and four of
-// 16-byte demons, double-buffered, per thread slot.
-static size_t pair_lds(const Geo &g, uint32_t ty) { return (size_t)128 * (ty + 6) * g.Lh; }
+// LDS of the sweep-pair kernel per thread slot: four double-buffered planes
+// of 8-byte words and two three-slot rings of 16-byte demons.
+static size_t pair_lds(const Geo &g, uint32_t ty) { return (size_t)160 * (ty + 6) * g.Lh; }
 constexpr size_t kPairLdsMax = 160 * 1024;
 
 // A sweep-pair tile is valid for one demon layer (its demons walk along x,
