@@ -597,8 +597,8 @@ __global__ void __launch_bounds__(1024) k_sweep_pair(PairArgs a) {
   const bool rowB1 = row >= 3 && row + 3 < ny;
   const bool hrow = row == 0 || row + 1 == ny;
   const uint32_t P = ny * Lh, slot = row * Lh + i;
-  uint64_t *S1p = s_lds, *A0b = s_lds + 2 * P, *A1b = s_lds + 4 * P, *B0b = s_lds + 6 * P;
-  ulonglong2 *XD0 = reinterpret_cast<ulonglong2 *>(s_lds + 8 * P), *XD1 = XD0 + 3 * P;
+  uint64_t *S1p = s_lds, *A0b = s_lds + 2 * P, *A1b = s_lds + 5 * P, *B0b = s_lds + 8 * P;
+  ulonglong2 *XD0 = reinterpret_cast<ulonglong2 *>(s_lds + 11 * P), *XD1 = XD0 + 3 * P;
   const uint64_t *S0in = a.sp_in + (uint64_t)(2 * r) * H, *S1in = S0in + H;
   uint64_t *S0out = a.sp_out + (uint64_t)(2 * r) * H, *S1out = S0out + H;
   const ulonglong2 *Din0 = a.dm_in + (uint64_t)(2 * r) * H, *Din1 = Din0 + H;
@@ -613,9 +613,7 @@ __global__ void __launch_bounds__(1024) k_sweep_pair(PairArgs a) {
   uint64_t Sm = ldo(S1in, (i + yb + LhL * wrapi((int)z0 - 4, L)) * 8u);
   uint64_t S0w = ldo(S1in, (i + yb + LhL * wrapi((int)z0 - 3, L)) * 8u);
   uint64_t Sp = 0;
-  uint64_t E0 = 0, Em1 = 0, Em2 = 0;      // A0 results, planes s, s-1, s-2
-  uint64_t F0 = 0, F1 = 0, F2 = 0;        // A1 results, planes s-1, s-2, s-3
-  uint64_t G0 = 0, G1 = 0, G2 = 0;        // B0 results, planes s-2, s-3, s-4
+  uint64_t E0 = 0, F0 = 0, G0 = 0;        // A0, A1, B0 results of the step (planes s, s-1, s-2)
   // J windows, [d] = loaded d+1 steps ago: the first sweep's own bonds
   // (colour 0 at plane s-1-d: x, y, z; colour 1: x, y, z), the colour-1 bonds
   // of A0's -x and -y neighbours and the colour-0 bonds of A1's
@@ -632,8 +630,9 @@ __global__ void __launch_bounds__(1024) k_sweep_pair(PairArgs a) {
     const uint32_t h2 = i + yb + LhL * z2, h3 = i + yb + LhL * z3;
     const uint32_t off = (y + z) & 1u;
     const uint32_t ip = off ? (i + 1 == Lh ? 0 : i + 1) : i, im = off ? i : (i == 0 ? Lh - 1 : i - 1);
-    const int cur = k & 1, prv = cur ^ 1;
-    const uint32_t ring = (uint32_t)(k % 3), ring2 = (uint32_t)((k + 1) % 3);   // slots of k and k-2
+    const int cur = k & 1;
+    // three-slot rings: the results of steps k, k-1 and k-2
+    const uint32_t ring = (uint32_t)(k % 3), ring1 = (uint32_t)((k + 2) % 3), ring2 = (uint32_t)((k + 1) % 3);
     // ---- global loads of the step and the LDS write before the barrier
     Sp = ldo(S1in, (i + yb + LhL * wrapi(s + 1, L)) * 8u);
     const uint64_t j0x = ldo(Jx0, h * 8u), j0y = ldo(Jy0, h * 8u), j0z = ldo(Jz0, h * 8u);
@@ -653,7 +652,7 @@ __global__ void __launch_bounds__(1024) k_sweep_pair(PairArgs a) {
       const uint64_t nym = row == 0 ? Hv : Sl[rm * Lh + i];
       E0 = msa::site_update(S0v, nxp, nyp, Sp, nxm, nym, Sm, j0x, j0y, j0z, c0J3, c0J4, w1z[0],
                             D0, D1);
-      A0b[cur * P + slot] = E0;
+      A0b[ring * P + slot] = E0;
       XD0[ring * P + row * Lh + iw0] = make_ulonglong2(D0, D1);   // walked, read by B0 at k+2
     }
     // ---- A1: first sweep, colour 1, plane s-1
@@ -663,22 +662,25 @@ __global__ void __launch_bounds__(1024) k_sweep_pair(PairArgs a) {
       c1J4 = ldo(Jy0, (i + ymb + LhL * z1) * 8u);
       const ulonglong2 d = ldo_stream(Din1, h1 * 16u);
       uint64_t D0 = d.x, D1 = d.y;
-      const uint64_t *El = A0b + prv * P;
+      const uint64_t *El = A0b + ring1 * P;
+      const uint64_t Em2 = A0b[ring2 * P + slot];   // A0 at plane s-2, own column
       F0 = msa::site_update(Sm, El[row * Lh + ip], El[rp * Lh + i], E0, El[row * Lh + im],
                             El[rm * Lh + i], Em2, w1x[0], w1y[0], w1z[0], c1J3, c1J4, w0z[1],
                             D0, D1);
-      A1b[cur * P + slot] = F0;
+      A1b[ring * P + slot] = F0;
       XD1[ring * P + row * Lh + iw1] = make_ulonglong2(D0, D1);   // walked, read by B1 at k+2
     }
     // ---- B0: second sweep, colour 0, plane s-2 (J of A0 two steps ago)
     if (rowB0 && k >= 4) {
       const ulonglong2 d = XD0[ring2 * P + slot];
       uint64_t D0 = d.x, D1 = d.y;
-      const uint64_t *Fl = A1b + prv * P;
+      const uint64_t *Fl = A1b + ring1 * P;
+      const uint64_t Em2 = A0b[ring2 * P + slot];   // own spin: A0 at plane s-2
+      const uint64_t F2 = A1b[ring2 * P + slot];    // A1 at plane s-3, own column
       G0 = msa::site_update(Em2, Fl[row * Lh + ip], Fl[rp * Lh + i], F0, Fl[row * Lh + im],
                             Fl[rm * Lh + i], F2, w0x[1], w0y[1], w0z[1], wc0x[1], wc0y[1], w1z[2],
                             D0, D1);
-      B0b[cur * P + slot] = G0;
+      B0b[ring * P + slot] = G0;
       if (rowB1 && k >= 5 && k <= (int)a.tz + 4) {
         sto_stream(S0out, h2 * 8u, G0);
         sto_stream(Dout0, (iw0 + yb + LhL * z2) * 16u, D0, D1);
@@ -688,7 +690,9 @@ __global__ void __launch_bounds__(1024) k_sweep_pair(PairArgs a) {
     if (rowB1 && k >= 6) {
       const ulonglong2 d = XD1[ring2 * P + slot];
       uint64_t D0 = d.x, D1 = d.y;
-      const uint64_t *Gl = B0b + prv * P;
+      const uint64_t *Gl = B0b + ring1 * P;
+      const uint64_t F2 = A1b[ring2 * P + slot];    // own spin: A1 at plane s-3
+      const uint64_t G2 = B0b[ring2 * P + slot];    // B0 at plane s-4, own column
       const uint64_t sn = msa::site_update(F2, Gl[row * Lh + ip], Gl[rp * Lh + i], G0,
                                            Gl[row * Lh + im], Gl[rm * Lh + i], G2, w1x[2], w1y[2],
                                            w1z[2], wc1x[1], wc1y[1], w0z[3], D0, D1);
@@ -698,12 +702,6 @@ __global__ void __launch_bounds__(1024) k_sweep_pair(PairArgs a) {
     // ---- rotate the windows
     Sm = S0w;
     S0w = Sp;
-    Em2 = Em1;
-    Em1 = E0;
-    F2 = F1;
-    F1 = F0;
-    G2 = G1;
-    G1 = G0;
     w0x[1] = w0x[0]; w0x[0] = j0x;
     w0y[1] = w0y[0]; w0y[0] = j0y;
     w0z[3] = w0z[2]; w0z[2] = w0z[1]; w0z[1] = w0z[0]; w0z[0] = j0z;
@@ -874,9 +872,10 @@ static bool narrow_ok(const Geo &g) {
   return 16ull * g.R * g.nc * g.V < lim && 24ull * g.V < lim && 8ull * g.R * g.V < lim;
 }
 
-// LDS of the sweep-pair kernel per thread slot: four double-buffered planes
-// of 8-byte words and two three-slot rings of 16-byte demons.
-static size_t pair_lds(const Geo &g, uint32_t ty) { return (size_t)160 * (ty + 6) * g.Lh; }
+// LDS of the sweep-pair kernel per thread slot: the odd input spins
+// double-buffered, three three-slot rings of 8-byte results and two of
+// 16-byte demons.
+static size_t pair_lds(const Geo &g, uint32_t ty) { return (size_t)184 * (ty + 6) * g.Lh; }
 constexpr size_t kPairLdsMax = 160 * 1024;
 
 // A sweep-pair tile is valid for one demon layer (its demons walk along x,
