@@ -641,6 +641,11 @@ __global__ void __launch_bounds__(1024) k_sweep_pair(PairArgs a) {
     const uint64_t S0v = ldo(S0in, h * 8u);
     const ulonglong2 dA0 = ldo_stream(Din0, h * 16u);
     const uint64_t Hv = hrow ? ldo(S1in, (i + hrb + LhL * z) * 8u) : 0;
+    // A1's loads (its -x and -y colour-0 bonds, its demons) with the others
+    const bool doA1 = rowA1 && k >= 2;
+    const uint64_t c1J3 = doA1 ? ldo(Jx0, (im + yb + LhL * z1) * 8u) : 0;
+    const uint64_t c1J4 = doA1 ? ldo(Jy0, (i + ymb + LhL * z1) * 8u) : 0;
+    const ulonglong2 dA1 = doA1 ? ldo_stream(Din1, h1 * 16u) : make_ulonglong2(0, 0);
     S1p[cur * P + slot] = S0w;
     __syncthreads();
     // ---- A0: first sweep, colour 0, plane s
@@ -656,12 +661,8 @@ __global__ void __launch_bounds__(1024) k_sweep_pair(PairArgs a) {
       XD0[ring * P + row * Lh + iw0] = make_ulonglong2(D0, D1);   // walked, read by B0 at k+2
     }
     // ---- A1: first sweep, colour 1, plane s-1
-    uint64_t c1J3 = 0, c1J4 = 0;
-    if (rowA1 && k >= 2) {
-      c1J3 = ldo(Jx0, (im + yb + LhL * z1) * 8u);
-      c1J4 = ldo(Jy0, (i + ymb + LhL * z1) * 8u);
-      const ulonglong2 d = ldo_stream(Din1, h1 * 16u);
-      uint64_t D0 = d.x, D1 = d.y;
+    if (doA1) {
+      uint64_t D0 = dA1.x, D1 = dA1.y;
       const uint64_t *El = A0b + ring1 * P;
       const uint64_t Em2 = A0b[ring2 * P + slot];   // A0 at plane s-2, own column
       F0 = msa::site_update(Sm, El[row * Lh + ip], El[rp * Lh + i], E0, El[row * Lh + im],
