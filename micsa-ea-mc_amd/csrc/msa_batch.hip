@@ -64,12 +64,12 @@ __device__ __forceinline__ uint32_t div_recip(uint32_t n, uint32_t m) { return _
 // it stages a lattice's two search tables in LDS when its run reaches that
 // lattice and the lattice refreshes this iteration.  Index arithmetic is
 // 32-bit (every batched lattice has L^3 < 2^31, checked on admission) and
-// divisions by L/2 and L use per-lattice reciprocals.
+// divisions by L/2 and L use per-lattice reciprocals.  Held to four waves
+// per SIMD (128 VGPRs), the occupancy the tables' LDS allows anyway.
 template <int BS>
-__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) k_batch_halfsweep(const BLat *__restrict__ lats, BProg *prog,
-                                                        const BSeg *__restrict__ segs,
-                                                        const uint32_t *__restrict__ seg_start,
-                                                        int colour) {
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4)))
+k_batch_halfsweep(const BLat *__restrict__ lats, BProg *prog, const BSeg *__restrict__ segs,
+                  const uint32_t *__restrict__ seg_start, int colour) {
   extern __shared__ uint4 s_tab[];
   const msa::FineLut *tabs = reinterpret_cast<const msa::FineLut *>(s_tab);
   const msa::LutView t0 = msa::lut_view(tabs), t1 = msa::lut_view(tabs + 1);

@@ -142,6 +142,14 @@ __device__ __forceinline__ uint32_t bisect(const LutView &t, uint32_t rm, uint32
   return below(lo) ? lo : hi;
 }
 
+// Rotate left by r in 0..31, the last step of _get_rnd_word (main.cu:644;
+// for r = 0 its right shift by 32 gives 0 on the reference's GPUs, so w is
+// left as it is): one v_alignbit of (w:w) by 32 - r, whose shift count is
+// taken mod 32, so r = 0 shifts by 0.
+__device__ __forceinline__ uint32_t rotl32(uint32_t w, uint32_t r) {
+  return __builtin_amdgcn_alignbit(w, w, 32u - r);
+}
+
 // _obten_aleatorio_contador + _get_rnd_word (main.cu:597-644): one Philox
 // call -> four 64-bit uniforms -> four bytes of Bernoulli bits, rotated.
 // The counter's x is incremented after the call (main.cu:606).
@@ -160,7 +168,7 @@ __device__ __forceinline__ uint32_t rnd_word(const LutView &t, uint32_t c[4], ui
   w |= bisect(t, v1, l1) << 8;
   w |= bisect(t, v2, l2) << 16;
   w |= bisect(t, v3, l3) << 24;
-  return rot ? ((w << rot) | (w >> (32u - rot))) : w;
+  return rotl32(w, rot);
 }
 
 // The search of bisect() inside a bracket [lo, hi] known to hold the answer
@@ -216,7 +224,7 @@ __device__ __forceinline__ void rnd_word_pair(const LutView &t, uint32_t cx, uin
 #pragma unroll
   for (int j = 0; j < 2; ++j) {
     const uint32_t x = lo[4 * j] | (lo[4 * j + 1] << 8) | (lo[4 * j + 2] << 16) | (lo[4 * j + 3] << 24);
-    w[j] = rot[j] ? ((x << rot[j]) | (x >> (32u - rot[j]))) : x;
+    w[j] = rotl32(x, rot[j]);
   }
   wa = w[0];
   wb = w[1];

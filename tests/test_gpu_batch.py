@@ -123,6 +123,41 @@ def test_batch_runs_span_lattices(pkg, orc, block):
     batch.close()
 
 
+def test_batch_planned_progress_and_early_reads(pkg, orc):
+    """The host's planned counts equal the device's after a sync; a lattice
+    that finished in a run can be read (on its own stream) while the next run
+    is still queued, and its state is the final one."""
+    a = lattice_case(orc, 16, 1, 6, 1, 21)
+    b = lattice_case(orc, 24, 1, 40, 1, 22)
+    raw = a[3] + b[3]
+    batch = pkg.Batch(raw)
+    batch.set_shape(256, 4)
+    sa = batch.add(pkg.Batch.make_job(16, 1, 5, 6, 1, a[1], a[2], np.arange(len(a[5]), dtype=np.int32)))
+    sb = batch.add(pkg.Batch.make_job(24, 1, 6, 40, 1, b[1], b[2],
+                                      np.arange(len(b[5]), dtype=np.int32) + len(a[5])))
+    batch.upload(sa, a[0].J, a[0].S, a[0].D0, a[0].D1)
+    batch.upload(sb, b[0].J, b[0].S, b[0].D0, b[0].D1)
+    batch.run(8)                                  # a finishes at 6, b reaches 8
+    t, te = batch.planned(2)
+    assert list(t) == [6, 8] and list(te) == [6, 40]
+    batch.run(32)                                 # queued behind the first run
+    batch.wait_previous()                         # the first run is done
+    early = batch.download(sa)                    # a is final; b's run may still be going
+    e_early = batch.measure(sa)["energy"]
+    batch.sync()
+    t2, _ = batch.progress(2)
+    assert list(t2) == [6, 40] and list(batch.planned(2)[0]) == [6, 40]
+    for x, y in zip(early, batch.download(sa)):
+        assert np.array_equal(x, y)
+    # the oracle on lattice a: six iterations with refreshes every iteration
+    o, ctr, keys, _, luts, times = a
+    for tt in range(6):
+        orc.iteration(o, ctr[times.index(tt)], keys, luts[times.index(tt)])
+    assert np.array_equal(early[0], o.S) and np.array_equal(early[1], o.D0)
+    assert np.array_equal(e_early, orc.energy(o))
+    batch.close()
+
+
 def test_batch_admission_and_churn(pkg, orc):
     o, ctr, keys, raw, luts, times = lattice_case(orc, 16, 1, 12, 1, 3)
     job = pkg.Batch.make_job(16, 1, 77, 12, 1, ctr, keys, np.arange(len(times), dtype=np.int32))
