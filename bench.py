@@ -110,6 +110,11 @@ def cpu_baseline(L, R, nc, budget_s, threads):
                       "%.1f s" % (n, L, R, dt)}
 
 
+def progress(msg):
+    """A progress line on stderr (stdout carries only the JSON result)."""
+    print("bench.py: " + msg, file=sys.stderr, flush=True)
+
+
 def splitmix64(x):
     x = (x + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
     z = x
@@ -259,9 +264,10 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--batched", type=int, default=48, help="lattices in the batched test (0: skip)")
-    ap.add_argument("--batched-len-div", type=int, default=64)
-    ap.add_argument("--served", type=int, default=60, help="requests in the served test (0: skip)")
+    ap.add_argument("--batched", type=int, default=512, help="lattices in the batched test (0: skip)")
+    ap.add_argument("--batched-len-div", type=int, default=16,
+                    help="anneal lengths {4096,16384,65536} divided by this")
+    ap.add_argument("--served", type=int, default=1000, help="requests in the served test (0: skip)")
     ap.add_argument("--served-load", type=float, default=0.8,
                     help="arrival rate as a fraction of the measured batched capacity")
     ap.add_argument("--no-autotune", action="store_true",
@@ -412,10 +418,14 @@ def main():
     if copy is not None:
         out["copy_probe"] = copy
     if args.batched > 0:
+        progress("headline lattice done; batched form: %d lattices" % args.batched)
         out["batched"] = batched_vs_plain(pkg, args.batched, args.batched_len_div)
     if args.served > 0:
+        progress("served form: %d requests" % args.served)
+        # lengths 2^6..2^11: the served configuration's 2^12..2^17 sweeps / 64
         out["served"] = served(pkg, args.served, args.served_load, (6, 11), 8 << 30)
     if not args.no_cpu:
+        progress("CPU baseline: %.0f s" % args.cpu_budget)
         out["cpu_baseline"] = cpu_baseline(L, R, nc, args.cpu_budget, args.cpu_threads)
     print(json.dumps(out))
 

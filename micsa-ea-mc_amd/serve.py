@@ -148,49 +148,51 @@ def serve(requests, max_bytes, epoch=64, block=None, tables=None, keep_state=Fal
         updates += 64 * req.R * req.L ** 3 * req.niter
         batch.remove(slot)
 
-    while len(done) < len(reqs):
-        now = clock() - t_start
-        while nxt < len(reqs) and reqs[nxt].arrival <= now:
-            req = reqs[nxt]
-            prepared[req.rid] = pool.submit(tables.job, req, req.rid)
-            queue.append(req)
-            nxt += 1
-        while queue:                              # FIFO admission by footprint
-            req = queue[0]
-            fut = prepared[req.rid]
-            if not fut.done() and active:
-                break                             # admitted at a later epoch
-            slot = batch.add(fut.result())
-            if slot is None:
-                if not active:
-                    raise RuntimeError("request %d alone exceeds the memory limit" % req.rid)
-                break
-            queue.pop(0)
-            del prepared[req.rid]
-            active[slot] = (req, clock() - t_start)
-        max_active = max(max_active, len(active))
-        max_bytes_used = max(max_bytes_used, batch.bytes_in_use())
-        running = [s for s in active if s not in finishing]
-        if not running and not finishing:
-            if nxt < len(reqs):
-                time.sleep(max(0.0, min(0.01, reqs[nxt].arrival - (clock() - t_start))))
-            continue
-        # one epoch stays queued on the device while the previous one's
-        # finished requests are measured and retired
-        new_finishing = []
-        if running:
-            batch.run(epoch)
-            t, tend = batch.planned(max(running) + 1)
-            new_finishing = [s for s in running if t[s] >= tend[s]]
-            batch.wait_previous()
-        else:
-            batch.sync()
-        for slot in finishing:
-            retire(slot)
-        finishing = new_finishing
-    makespan = clock() - t_start
-    pool.shutdown()
-    batch.close()
+    try:
+        while len(done) < len(reqs):
+            now = clock() - t_start
+            while nxt < len(reqs) and reqs[nxt].arrival <= now:
+                req = reqs[nxt]
+                prepared[req.rid] = pool.submit(tables.job, req, req.rid)
+                queue.append(req)
+                nxt += 1
+            while queue:                              # FIFO admission by footprint
+                req = queue[0]
+                fut = prepared[req.rid]
+                if not fut.done() and active:
+                    break                             # admitted at a later epoch
+                slot = batch.add(fut.result())
+                if slot is None:
+                    if not active:
+                        raise RuntimeError("request %d alone exceeds the memory limit" % req.rid)
+                    break
+                queue.pop(0)
+                del prepared[req.rid]
+                active[slot] = (req, clock() - t_start)
+            max_active = max(max_active, len(active))
+            max_bytes_used = max(max_bytes_used, batch.bytes_in_use())
+            running = [s for s in active if s not in finishing]
+            if not running and not finishing:
+                if nxt < len(reqs):
+                    time.sleep(max(0.0, min(0.01, reqs[nxt].arrival - (clock() - t_start))))
+                continue
+            # one epoch stays queued on the device while the previous one's
+            # finished requests are measured and retired
+            new_finishing = []
+            if running:
+                batch.run(epoch)
+                t, tend = batch.planned(max(running) + 1)
+                new_finishing = [s for s in running if t[s] >= tend[s]]
+                batch.wait_previous()
+            else:
+                batch.sync()
+            for slot in finishing:
+                retire(slot)
+            finishing = new_finishing
+        makespan = clock() - t_start
+    finally:
+        pool.shutdown()
+        batch.close()
     lat = np.array([done[r.rid]["latency"] for r in reqs])
     summary = {"requests": len(reqs), "makespan_s": makespan,
                "spin_updates_per_ps": updates / makespan / 1e12,

@@ -255,3 +255,31 @@ def test_sweep_pairs_agree_at_larger_sizes(pkg, L, R, ty, tz):
         assert np.array_equal(x, y)
     for d in devs:
         d.close()
+
+
+def test_L16_anneal_configuration(pkg, orc):
+    """The L = 16 parity configuration end to end: 64 +-J samples, R = 1,
+    random J and spins, demons drawn at T = 2.0 by the first refresh, linear
+    anneal T 2.0 -> 0.7 over 4096 sweeps with a refresh every sweep, per-lane
+    integer energy every 64 sweeps; every measurement and the final state
+    bit-exact against the oracle."""
+    L, R, nc, nsweeps, every = 16, 1, 1, 4096, 64
+    o = random_state(L, R, nc, 16016)
+    o.D0[...] = 0
+    o.D1[...] = 0
+    times = list(range(nsweeps))
+    betas = [beta_of(t, nsweeps) for t in times]
+    ctr, keys, raw, luts = oracle_schedule(orc, R, times, betas, jobid=1)
+    dev = pkg.Lattice(L, R, nc)
+    dev.upload(o.J, o.S, o.D0, o.D1)
+    dev.set_luts(raw)
+    dev.set_refresh(ctr, keys, np.arange(nsweeps, dtype=np.int32))
+    for t0 in range(0, nsweeps, every):
+        dev.run(t0, every, 1)
+        for t in range(t0, t0 + every):
+            orc.iteration(o, ctr[t], keys, luts[t], nthreads=8)
+        m = dev.measure()
+        assert np.array_equal(m["energy"], orc.energy(o)), "energy differs after %d sweeps" % (t0 + every)
+        assert np.array_equal(m["demon"], orc.demon_energy(o))
+    assert_same_state(dev, o, "after %d sweeps" % nsweeps)
+    dev.close()
