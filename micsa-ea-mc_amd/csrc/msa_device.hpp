@@ -234,19 +234,13 @@ __device__ __forceinline__ void rnd_word_pair(const LutView &t, uint32_t cx, uin
 // thread id tid = layer*V + site, counter ctr with .z += tid, key {tid, key},
 // two words from the plane-0 table then two from the plane-1 table (rnd_word
 // four times, the counter's x advancing after each).  refresh_plane(p) is
-// plane p alone (D0 for p = 0, D1 for p = 1), so callers can place other
-// work between the two.
+// plane p (D0 for p = 0 from the plane-0 table, D1 for p = 1 from the
+// plane-1 table), so callers can place other work between the two.
 __device__ __forceinline__ uint64_t refresh_plane(const LutView &t, uint4 ctr, uint32_t key,
                                                   uint32_t tid, int plane) {
   uint32_t wa, wb;
   rnd_word_pair(t, ctr.x + 2u * (uint32_t)plane, ctr.y, ctr.z + tid, ctr.w, tid, key, wa, wb);
   return ((uint64_t)wa << 32) | wb;
-}
-__device__ __forceinline__ void refresh_demon(const LutView &t0, const LutView &t1, uint4 ctr,
-                                              uint32_t key, uint32_t tid, uint64_t &d0,
-                                              uint64_t &d1) {
-  d0 = refresh_plane(t0, ctr, key, tid, 0);
-  d1 = refresh_plane(t1, ctr, key, tid, 1);
 }
 
 // ------------------------------------------------------------ the update --
