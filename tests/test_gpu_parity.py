@@ -283,3 +283,27 @@ def test_L16_anneal_configuration(pkg, orc):
         assert np.array_equal(m["demon"], orc.demon_energy(o))
     assert_same_state(dev, o, "after %d sweeps" % nsweeps)
     dev.close()
+
+
+def test_L64_R2_configuration_start(pkg, orc):
+    """The start of the L = 64 parity configuration: R = 2 replicas sharing
+    J, anneal T 2.0 -> 0.7 over 65536 sweeps with a refresh every sweep; the
+    first 256 sweeps, with the integer energies, demon energies, q and q_link
+    of the replica pair compared bit-exactly every 128 sweeps."""
+    L, R, nc, nsweeps, every = 64, 2, 1, 256, 128
+    o = random_state(L, R, nc, 64064)
+    o.D0[...] = 0
+    o.D1[...] = 0
+    times = list(range(nsweeps))
+    ctr, keys, raw, luts = oracle_schedule(orc, R, times, [beta_of(t, 65536) for t in times], jobid=2)
+    dev = pkg.Lattice(L, R, nc)
+    dev.upload(o.J, o.S, o.D0, o.D1)
+    dev.set_luts(raw)
+    dev.set_refresh(ctr, keys, np.arange(nsweeps, dtype=np.int32))
+    for t0 in range(0, nsweeps, every):
+        dev.run(t0, every, 1)
+        for t in range(t0, t0 + every):
+            orc.iteration(o, ctr[t], keys, luts[t], nthreads=16)
+        assert_same_measures(dev, orc, o)
+    assert_same_state(dev, o, "after %d sweeps" % nsweeps)
+    dev.close()
