@@ -372,19 +372,31 @@ struct VCount {
     }
   }
   // Sum the counters of the 64 threads of the wavefront; lane b returns the
-  // total for sample b.  Each ballot transposes one bit-plane across the wave.
+  // total for sample b.  Every lane of the wavefront must be active.
   __device__ __forceinline__ uint32_t wave_total() const {
-    const uint32_t lane = __lane_id();
     uint32_t acc = 0;
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
       if (__ballot(c[j] != 0) == 0) continue;
-      for (int b = 0; b < 64; ++b) {
-        const uint64_t m = __ballot((int)((c[j] >> b) & 1ull));
-        if (lane == (uint32_t)b) acc += (uint32_t)__popcll(m) << j;
-      }
+      acc += (uint32_t)__popcll(wave_transpose(c[j])) << j;
     }
     return acc;
+  }
+  // Transpose the 64 x 64 bit matrix held one row per lane (bit b of lane t
+  // becomes bit t of lane b): six stages, each swapping the off-diagonal
+  // s x s blocks of every 2s x 2s block between lanes t and t ^ s.
+  static __device__ __forceinline__ uint64_t wave_transpose(uint64_t x) {
+    const uint32_t lane = __lane_id();
+    constexpr uint64_t keep[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull, 0x00FF00FF00FF00FFull,
+                                  0x0F0F0F0F0F0F0F0Full, 0x3333333333333333ull, 0x5555555555555555ull};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const int s = 32 >> k;
+      const uint64_t m = keep[k];
+      const uint64_t y = __shfl_xor(x, s);
+      x = (lane & (uint32_t)s) ? ((x & ~m) | ((y >> s) & m)) : ((x & m) | ((y & m) << s));
+    }
+    return x;
   }
 };
 

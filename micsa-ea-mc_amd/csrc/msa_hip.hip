@@ -717,72 +717,120 @@ __global__ void __launch_bounds__(1024) k_sweep_pair(PairArgs a) {
 }
 
 struct MeasArgs {
-  const uint64_t *sp;
-  const uint64_t *jp;
-  const ulonglong2 *dm;
+  const uint64_t *sp;     // spins [2R][H]: replica r colour c in plane 2r + c
+  const uint64_t *jp;     // couplings [2 colours][3 directions][H]
+  const ulonglong2 *dm;   // demons [R][nc][2 colours][H]
   Geo g;
-  int what;       // 0 unsat bonds of replica ra, 1 demon energy of ra, 2 q(ra,rb), 3 qlink(ra,rb)
-  int ra, rb;
-  uint32_t per_thread;
-  int32_t *out;   // [64]
+  uint32_t nr, mLh;       // rows of L/2 words per step; div_recip reciprocal of L/2
+  uint32_t steps;         // row steps per block (counter widths hold 16)
+  int32_t *out;           // [R unsat | R demon | R/2 q | R/2 qlink][64]
 };
 
-// Per-lane integer observables.  Every bond touches exactly one colour-0
-// site, so the six bonds of the colour-0 sites cover the lattice once
-// (the sum d_computeEne forms, main.cu:745-815).  Counts accumulate in
-// bit-sliced counters and are transposed to lanes with wavefront ballots.
-__global__ void __launch_bounds__(256) k_measure(MeasArgs a) {
+// Every per-lane integer observable of the lattice in one pass.  Every bond
+// touches exactly one colour-0 site, so the six bonds of the colour-0 sites
+// cover the lattice once (the sum d_computeEne forms, main.cu:745-815); the
+// demon energy is Σ D0 + 2 D1 over both colours and all layers
+// (Measure_Energy_Clerics, med.c:55-83).  blockIdx.y = 2p + k: replicas 2p
+// and 2p + 1, k = 0 the spin observables (unsatisfied bonds of each, q and
+// q_link of the pair), k = 1 the demon energies.  Counts accumulate in
+// bit-sliced counters, are transposed to lanes across the wavefront, summed
+// over the block in LDS and added to the output once per block.
+__global__ void __launch_bounds__(512) k_measure(MeasArgs a) {
+  __shared__ uint32_t bsum[4][64];
   const Geo g = a.g;
-  msa::VCount<10> cnt;
-  cnt.clear();
-  const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint64_t nthr = (uint64_t)gridDim.x * blockDim.x;
-  for (uint32_t k = 0; k < a.per_thread; ++k) {
-    const uint64_t h = gid + k * nthr;
-    if (h >= g.H) break;
-    const uint32_t i = (uint32_t)(h % g.Lh);
-    const uint64_t yz = h / g.Lh;
-    const uint32_t y = (uint32_t)(yz % g.L), z = (uint32_t)(yz / g.L);
-    if (a.what == 1) {
-      for (uint32_t l = 0; l < g.nc; ++l)
-        for (int c = 0; c < 2; ++c) {
-          const ulonglong2 d = a.dm[((uint64_t)(a.ra * g.nc + l) * 2 + c) * g.H + h];
-          cnt.add(d.x, 0);
-          cnt.add(d.y, 1);
-        }
-      continue;
-    }
-    if (a.what == 2) {
-      for (int c = 0; c < 2; ++c)
-        cnt.add(a.sp[(uint64_t)(2 * a.ra + c) * g.H + h] ^ a.sp[(uint64_t)(2 * a.rb + c) * g.H + h]);
-      continue;
-    }
-    const uint32_t off = (y + z) & 1u;   // colour-0 site
-    const uint32_t ip = off ? msa::wrapi((int)i + 1, g.Lh) : i;
-    const uint32_t im = off ? i : msa::wrapi((int)i - 1, g.Lh);
-    const uint32_t yp = msa::wrapi((int)y + 1, g.L), ym = msa::wrapi((int)y - 1, g.L);
-    const uint32_t zp = msa::wrapi((int)z + 1, g.L), zm = msa::wrapi((int)z - 1, g.L);
-    const uint64_t row = (uint64_t)g.Lh * (y + (uint64_t)g.L * z);
-    const uint64_t nb[6] = {ip + row, i + (uint64_t)g.Lh * (yp + (uint64_t)g.L * z),
-                            i + (uint64_t)g.Lh * (y + (uint64_t)g.L * zp), im + row,
-                            i + (uint64_t)g.Lh * (ym + (uint64_t)g.L * z),
-                            i + (uint64_t)g.Lh * (y + (uint64_t)g.L * zm)};
-    const uint64_t *Sa0 = a.sp + (uint64_t)(2 * a.ra) * g.H, *Sa1 = Sa0 + g.H;
-    if (a.what == 0) {
-      const uint64_t *J0 = a.jp, *J1 = a.jp + 3 * g.H;
-      const uint64_t s = Sa0[h];
+  const uint32_t Lh = g.Lh, L = g.L, rows = L * L;
+  const uint64_t H = g.H;
+  const uint32_t rloc = div_recip(threadIdx.x, a.mLh), i = threadIdx.x - rloc * Lh;
+  const uint32_t p = blockIdx.y >> 1, ra = 2 * p, rb = ra + 1;
+  const bool pair = rb < g.R, demons = (blockIdx.y & 1u) != 0;
+  for (uint32_t k = threadIdx.x; k < 256; k += blockDim.x) (&bsum[0][0])[k] = 0;
+  uint32_t tot[4] = {0, 0, 0, 0};
+  if (!demons) {
+    msa::VCount<7> cu0, cu1, cq, cl;
+    cu0.clear();
+    cu1.clear();
+    cq.clear();
+    cl.clear();
+    const uint64_t *J0 = a.jp, *J1 = a.jp + 3 * H;
+    const uint64_t *Sa0 = a.sp + (uint64_t)(2 * ra) * H, *Sa1 = Sa0 + H;
+    const uint64_t *Sb0 = Sa1 + H, *Sb1 = Sb0 + H;
+    for (uint32_t k = 0; k < a.steps; ++k) {
+      const uint32_t row = (blockIdx.x * a.steps + k) * a.nr + rloc;
+      if (rloc >= a.nr || row >= rows) continue;
+      const uint32_t z = row / L, y = row - z * L;
+      const uint32_t off = (y + z) & 1u;   // colour-0 site at x = 2i + off
+      const uint32_t ip = off ? msa::wrapi((int)i + 1, Lh) : i;
+      const uint32_t im = off ? i : msa::wrapi((int)i - 1, Lh);
+      const uint32_t yp = msa::wrapi((int)y + 1, L), ym = msa::wrapi((int)y - 1, L);
+      const uint32_t zp = msa::wrapi((int)z + 1, L), zm = msa::wrapi((int)z - 1, L);
+      const uint64_t h = (uint64_t)row * Lh + i, rowb = (uint64_t)row * Lh;
+      const uint64_t nb[6] = {ip + rowb, i + (uint64_t)Lh * (yp + (uint64_t)L * z),
+                              i + (uint64_t)Lh * (y + (uint64_t)L * zp), im + rowb,
+                              i + (uint64_t)Lh * (ym + (uint64_t)L * z),
+                              i + (uint64_t)Lh * (y + (uint64_t)L * zm)};
+      uint64_t jv[6], na[6];
+#pragma unroll
       for (int d = 0; d < 3; ++d) {
-        cnt.add(s ^ J0[d * g.H + h] ^ Sa1[nb[d]]);
-        cnt.add(s ^ J1[d * g.H + nb[d + 3]] ^ Sa1[nb[d + 3]]);
+        jv[d] = J0[d * H + h];
+        jv[d + 3] = J1[d * H + nb[d + 3]];
       }
-    } else {
-      const uint64_t *Sb0 = a.sp + (uint64_t)(2 * a.rb) * g.H, *Sb1 = Sb0 + g.H;
-      const uint64_t sa = Sa0[h], sb = Sb0[h];
-      for (int d = 0; d < 6; ++d) cnt.add((sa ^ Sa1[nb[d]]) ^ (sb ^ Sb1[nb[d]]));
+#pragma unroll
+      for (int d = 0; d < 6; ++d) na[d] = Sa1[nb[d]];
+      const uint64_t sa = Sa0[h];
+#pragma unroll
+      for (int d = 0; d < 6; ++d) cu0.add(sa ^ jv[d] ^ na[d]);
+      if (pair) {
+        const uint64_t sb = Sb0[h];
+        cq.add(sa ^ sb);
+        cq.add(Sa1[h] ^ Sb1[h]);
+#pragma unroll
+        for (int d = 0; d < 6; ++d) {
+          const uint64_t nbv = Sb1[nb[d]];
+          cu1.add(sb ^ jv[d] ^ nbv);
+          cl.add((sa ^ na[d]) ^ (sb ^ nbv));
+        }
+      }
     }
+    tot[0] = cu0.wave_total();
+    tot[1] = cu1.wave_total();
+    tot[2] = cq.wave_total();
+    tot[3] = cl.wave_total();
+  } else {
+    msa::VCount<10> c0, c1;
+    c0.clear();
+    c1.clear();
+    const ulonglong2 *Da = a.dm + (uint64_t)ra * g.nc * 2 * H, *Db = Da + (uint64_t)g.nc * 2 * H;
+    for (uint32_t k = 0; k < a.steps; ++k) {
+      const uint32_t row = (blockIdx.x * a.steps + k) * a.nr + rloc;
+      if (rloc >= a.nr || row >= rows) continue;
+      const uint64_t h = (uint64_t)row * Lh + i;
+      for (uint32_t lc = 0; lc < 2 * g.nc; ++lc) {
+        const ulonglong2 d = Da[lc * H + h];
+        c0.add(d.x, 0);
+        c0.add(d.y, 1);
+        if (pair) {
+          const ulonglong2 e = Db[lc * H + h];
+          c1.add(e.x, 0);
+          c1.add(e.y, 1);
+        }
+      }
+    }
+    tot[0] = c0.wave_total();
+    tot[1] = c1.wave_total();
   }
-  const uint32_t tot = cnt.wave_total();
-  if (tot) atomicAdd(a.out + __lane_id(), (int32_t)tot);
+  __syncthreads();
+  const uint32_t lane = __lane_id();
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+    if (tot[j]) atomicAdd(&bsum[j][lane], tot[j]);
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k < 256; k += blockDim.x) {
+    const uint32_t j = k >> 6, b = k & 63u, v = bsum[j][b];
+    if (!v || (j > 0 && !pair) || (demons && j > 1)) continue;
+    const uint32_t slot = demons ? g.R + ra + j
+                                 : (j < 2 ? ra + j : 2 * g.R + (j == 2 ? 0 : g.R / 2) + p);
+    atomicAdd(a.out + 64 * slot + b, (int32_t)v);
+  }
 }
 
 // Counter-based construction (msa_init_philox): thread = canonical site.
@@ -821,11 +869,14 @@ static int check_geom(const msa_geom *g) {
 
 // Bytes of a context's state allocation: spins and demons twice (ping-pong),
 // J, the measurement counts, each rounded up to 256 bytes.
+// Rows of the count buffer: R unsatisfied-bond, R demon, R/2 q, R/2 q_link.
+static uint32_t meas_rows(uint32_t R) { return 2 * R + 2 * (R / 2); }
+
 static size_t state_bytes(const msa_geom *geom) {
   const size_t V = (size_t)geom->L * geom->L * geom->L;
   auto up = [](size_t n) { return (n + 255) & ~(size_t)255; };
   return 2 * up(8 * V * geom->R) + up(8 * V * 3) + 2 * up(16 * V * geom->R * geom->nc) +
-         up(64 * sizeof(int32_t));
+         up(64 * sizeof(int32_t) * meas_rows(geom->R));
 }
 
 extern "C" int64_t msa_footprint_bytes(const msa_geom *geom) {
@@ -1411,42 +1462,35 @@ extern "C" int msa_run_thermal(msa_ctx *c, int64_t niter, const msa_lut *lut, co
   return rc;
 }
 
-static int measure_one(msa_ctx *c, int what, int ra, int rb, int32_t *host_out) {
+extern "C" int msa_measure(msa_ctx *c, int32_t *unsat, int32_t *demon, int32_t *q,
+                           int32_t *qlink) {
+  if (!c) return msa_fail(MSA_ERR_ARG, "NULL context");
+  HIPCHK(hipSetDevice(c->device));
   const Geo &g = c->g;
-  const uint32_t per_thread = 8;
-  const uint64_t nthr = (g.H + per_thread - 1) / per_thread;
-  const uint32_t nb = (uint32_t)((nthr + 255) / 256);
+  const uint32_t nrow = meas_rows(g.R), rows = g.L * g.L;
   MeasArgs a{};
   a.sp = c->sp[c->scur];
   a.jp = c->jp;
   a.dm = c->dm[c->cur];
   a.g = g;
-  a.what = what;
-  a.ra = ra;
-  a.rb = rb;
-  a.per_thread = per_thread;
+  a.nr = g.Lh < 512 ? 512 / g.Lh : 1;   // blocks of at most 512 threads
+  a.mLh = div_recip_of(g.Lh);
+  a.steps = 4;
   a.out = c->meas;
-  HIPCHK(hipMemsetAsync(c->meas, 0, 64 * sizeof(int32_t), c->stream));
-  hipLaunchKernelGGL(k_measure, dim3(nb), dim3(256), 0, c->stream, a);
+  const uint32_t nb = (rows + a.nr * a.steps - 1) / (a.nr * a.steps);
+  const uint32_t threads = (a.nr * g.Lh + 63) / 64 * 64;   // whole wavefronts
+  std::vector<int32_t> host(64 * (size_t)nrow);
+  HIPCHK(hipMemsetAsync(c->meas, 0, host.size() * sizeof(int32_t), c->stream));
+  hipLaunchKernelGGL(k_measure, dim3(nb, 2 * ((g.R + 1) / 2)), dim3(threads), 0, c->stream, a);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(host_out, c->meas, 64 * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(host.data(), c->meas, host.size() * sizeof(int32_t), hipMemcpyDeviceToHost,
+                        c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  return MSA_OK;
-}
-
-extern "C" int msa_measure(msa_ctx *c, int32_t *unsat, int32_t *demon, int32_t *q,
-                           int32_t *qlink) {
-  if (!c) return msa_fail(MSA_ERR_ARG, "NULL context");
-  HIPCHK(hipSetDevice(c->device));
-  int rc;
-  for (uint32_t r = 0; r < c->g.R; ++r) {
-    if (unsat && (rc = measure_one(c, 0, (int)r, 0, unsat + 64 * r))) return rc;
-    if (demon && (rc = measure_one(c, 1, (int)r, 0, demon + 64 * r))) return rc;
-  }
-  for (uint32_t p = 0; 2 * p + 1 < c->g.R; ++p) {
-    if (q && (rc = measure_one(c, 2, (int)(2 * p), (int)(2 * p + 1), q + 64 * p))) return rc;
-    if (qlink && (rc = measure_one(c, 3, (int)(2 * p), (int)(2 * p + 1), qlink + 64 * p))) return rc;
-  }
+  const size_t nR = 64 * (size_t)g.R, nP = 64 * (size_t)(g.R / 2);
+  if (unsat) memcpy(unsat, host.data(), nR * sizeof(int32_t));
+  if (demon) memcpy(demon, host.data() + nR, nR * sizeof(int32_t));
+  if (q && nP) memcpy(q, host.data() + 2 * nR, nP * sizeof(int32_t));
+  if (qlink && nP) memcpy(qlink, host.data() + 2 * nR + nP, nP * sizeof(int32_t));
   return MSA_OK;
 }
 
