@@ -382,6 +382,27 @@ struct VCount {
     }
     return acc;
   }
+  // The value of lane t ^ s for s = 1, 2, 4, 8 (DPP moves: quad permutes,
+  // and row mirror / half-mirror, t ^ 15 and t ^ 7, composed with them),
+  // 16 (ds_swizzle, xor mask 16 within 32 lanes) and 32 (ds_bpermute).
+  static __device__ __forceinline__ uint32_t lane_xor32(uint32_t v, int s) {
+    constexpr int kXor1 = 0xB1, kXor2 = 0x4E, kXor3 = 0x1B, kMirror = 0x140, kHalfMirror = 0x141;
+    switch (s) {
+      case 1: return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, kXor1, 0xF, 0xF, false);
+      case 2: return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, kXor2, 0xF, 0xF, false);
+      case 4:
+        return (uint32_t)__builtin_amdgcn_mov_dpp(
+            __builtin_amdgcn_mov_dpp((int)v, kHalfMirror, 0xF, 0xF, false), kXor3, 0xF, 0xF, false);
+      case 8:
+        return (uint32_t)__builtin_amdgcn_mov_dpp(
+            __builtin_amdgcn_mov_dpp((int)v, kMirror, 0xF, 0xF, false), kHalfMirror, 0xF, 0xF, false);
+      case 16: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x401F);
+      default: return __shfl_xor(v, 32);
+    }
+  }
+  static __device__ __forceinline__ uint64_t lane_xor(uint64_t v, int s) {
+    return ((uint64_t)lane_xor32((uint32_t)(v >> 32), s) << 32) | lane_xor32((uint32_t)v, s);
+  }
   // Transpose the 64 x 64 bit matrix held one row per lane (bit b of lane t
   // becomes bit t of lane b): six stages, each swapping the off-diagonal
   // s x s blocks of every 2s x 2s block between lanes t and t ^ s.
@@ -393,7 +414,7 @@ struct VCount {
     for (int k = 0; k < 6; ++k) {
       const int s = 32 >> k;
       const uint64_t m = keep[k];
-      const uint64_t y = __shfl_xor(x, s);
+      const uint64_t y = lane_xor(x, s);
       x = (lane & (uint32_t)s) ? ((x & ~m) | ((y >> s) & m)) : ((x & m) | ((y & m) << s));
     }
     return x;

@@ -722,7 +722,7 @@ struct MeasArgs {
   const ulonglong2 *dm;   // demons [R][nc][2 colours][H]
   Geo g;
   uint32_t nr, mLh;       // rows of L/2 words per step; div_recip reciprocal of L/2
-  uint32_t steps;         // row steps per block (counter widths hold 16)
+  uint32_t steps;         // row steps per block, at most 16 (the counters' width)
   int32_t *out;           // [R unsat | R demon | R/2 q | R/2 qlink][64]
 };
 
@@ -1475,13 +1475,21 @@ extern "C" int msa_measure(msa_ctx *c, int32_t *unsat, int32_t *demon, int32_t *
   a.g = g;
   a.nr = g.Lh < 512 ? 512 / g.Lh : 1;   // blocks of at most 512 threads
   a.mLh = div_recip_of(g.Lh);
-  a.steps = 4;
   a.out = c->meas;
-  const uint32_t nb = (rows + a.nr * a.steps - 1) / (a.nr * a.steps);
+  // rows per thread: as many as leave one block per CU and job (fewer
+  // blocks finish their counters' transposes sooner), at most 16 (the
+  // width of the counters)
+  int ncu = 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || ncu < 1)
+    ncu = 256;
+  const uint32_t njob = 2 * ((g.R + 1) / 2), groups = (rows + a.nr - 1) / a.nr;
+  const uint32_t fill = (uint32_t)ncu / njob > 0 ? (uint32_t)ncu / njob : 1;
+  a.steps = groups / fill < 1 ? 1 : (groups / fill > 16 ? 16 : groups / fill);
+  const uint32_t nb = (groups + a.steps - 1) / a.steps;
   const uint32_t threads = (a.nr * g.Lh + 63) / 64 * 64;   // whole wavefronts
   std::vector<int32_t> host(64 * (size_t)nrow);
   HIPCHK(hipMemsetAsync(c->meas, 0, host.size() * sizeof(int32_t), c->stream));
-  hipLaunchKernelGGL(k_measure, dim3(nb, 2 * ((g.R + 1) / 2)), dim3(threads), 0, c->stream, a);
+  hipLaunchKernelGGL(k_measure, dim3(nb, njob), dim3(threads), 0, c->stream, a);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(host.data(), c->meas, host.size() * sizeof(int32_t), hipMemcpyDeviceToHost,
                         c->stream));
