@@ -354,6 +354,17 @@ __device__ __forceinline__ uint64_t metropolis_update(uint64_t s, uint64_t n0, u
 // --------------------------------------------------- per-lane counting --
 // Bit-sliced (vertical) counter: word j holds bit j of 64 independent lane
 // counts.  add() is a ripple-carry add of a 1-bit-per-lane word at weight 2^pos.
+// Per bit lane, the number of set bits among six words as b0 + 2 b1 + 4 b2
+// (two full adders, a half adder and a full adder over their carries).
+__device__ __forceinline__ void sum6(const uint64_t w[6], uint64_t &b0, uint64_t &b1, uint64_t &b2) {
+  const uint64_t s1 = w[0] ^ w[1] ^ w[2], c1 = (w[0] & w[1]) | (w[2] & (w[0] ^ w[1]));
+  const uint64_t s2 = w[3] ^ w[4] ^ w[5], c2 = (w[3] & w[4]) | (w[5] & (w[3] ^ w[4]));
+  const uint64_t c3 = s1 & s2;
+  b0 = s1 ^ s2;
+  b1 = c1 ^ c2 ^ c3;
+  b2 = (c1 & c2) | (c3 & (c1 ^ c2));
+}
+
 template <int NB>
 struct VCount {
   uint64_t c[NB];
@@ -369,6 +380,24 @@ struct VCount {
         c[j] ^= w;
         w = t;
       }
+    }
+  }
+  // Add the bit-sliced number b0 + 2 b1 + 4 b2 (a ripple of full adders).
+  __device__ __forceinline__ void add3(uint64_t b0, uint64_t b1, uint64_t b2) {
+    static_assert(NB >= 3, "add3 needs three bit-planes");
+    uint64_t carry = c[0] & b0;
+    c[0] ^= b0;
+    uint64_t s = c[1] ^ b1 ^ carry;
+    carry = (c[1] & b1) | (carry & (c[1] ^ b1));
+    c[1] = s;
+    s = c[2] ^ b2 ^ carry;
+    carry = (c[2] & b2) | (carry & (c[2] ^ b2));
+    c[2] = s;
+#pragma unroll
+    for (int j = 3; j < NB; ++j) {
+      const uint64_t t = c[j] & carry;
+      c[j] ^= carry;
+      carry = t;
     }
   }
   // Sum the counters of the 64 threads of the wavefront; lane b returns the

@@ -777,18 +777,27 @@ __global__ void __launch_bounds__(512) k_measure(MeasArgs a) {
 #pragma unroll
       for (int d = 0; d < 6; ++d) na[d] = Sa1[nb[d]];
       const uint64_t sa = Sa0[h];
+      uint64_t w[6], b0, b1, b2;
 #pragma unroll
-      for (int d = 0; d < 6; ++d) cu0.add(sa ^ jv[d] ^ na[d]);
+      for (int d = 0; d < 6; ++d) w[d] = sa ^ jv[d] ^ na[d];
+      msa::sum6(w, b0, b1, b2);
+      cu0.add3(b0, b1, b2);
       if (pair) {
         const uint64_t sb = Sb0[h];
-        cq.add(sa ^ sb);
-        cq.add(Sa1[h] ^ Sb1[h]);
+        const uint64_t q0 = sa ^ sb, q1 = Sa1[h] ^ Sb1[h];
+        cq.add3(q0 ^ q1, q0 & q1, 0);
+        uint64_t nbv[6];
 #pragma unroll
         for (int d = 0; d < 6; ++d) {
-          const uint64_t nbv = Sb1[nb[d]];
-          cu1.add(sb ^ jv[d] ^ nbv);
-          cl.add((sa ^ na[d]) ^ (sb ^ nbv));
+          nbv[d] = Sb1[nb[d]];
+          w[d] = sb ^ jv[d] ^ nbv[d];
         }
+        msa::sum6(w, b0, b1, b2);
+        cu1.add3(b0, b1, b2);
+#pragma unroll
+        for (int d = 0; d < 6; ++d) w[d] = (sa ^ na[d]) ^ (sb ^ nbv[d]);
+        msa::sum6(w, b0, b1, b2);
+        cl.add3(b0, b1, b2);
       }
     }
     tot[0] = cu0.wave_total();
@@ -804,15 +813,14 @@ __global__ void __launch_bounds__(512) k_measure(MeasArgs a) {
       const uint32_t row = (blockIdx.x * a.steps + k) * a.nr + rloc;
       if (rloc >= a.nr || row >= rows) continue;
       const uint64_t h = (uint64_t)row * Lh + i;
-      for (uint32_t lc = 0; lc < 2 * g.nc; ++lc) {
-        const ulonglong2 d = Da[lc * H + h];
-        c0.add(d.x, 0);
-        c0.add(d.y, 1);
-        if (pair) {
-          const ulonglong2 e = Db[lc * H + h];
-          c1.add(e.x, 0);
-          c1.add(e.y, 1);
-        }
+      // per layer, both colours: (x0 + x1) + 2 (y0 + y1) <= 6 as three bit-planes
+      auto layer_sum = [](const ulonglong2 &u, const ulonglong2 &v, msa::VCount<10> &cnt) {
+        const uint64_t s0 = u.x ^ v.x, k0 = u.x & v.x, s1 = u.y ^ v.y, k1 = u.y & v.y;
+        cnt.add3(s0, k0 ^ s1, (k0 & s1) | k1);
+      };
+      for (uint32_t l = 0; l < g.nc; ++l) {
+        layer_sum(Da[2 * l * H + h], Da[(2 * l + 1) * H + h], c0);
+        if (pair) layer_sum(Db[2 * l * H + h], Db[(2 * l + 1) * H + h], c1);
       }
     }
     tot[0] = c0.wave_total();
@@ -867,11 +875,11 @@ static int check_geom(const msa_geom *g) {
   return MSA_OK;
 }
 
-// Bytes of a context's state allocation: spins and demons twice (ping-pong),
-// J, the measurement counts, each rounded up to 256 bytes.
 // Rows of the count buffer: R unsatisfied-bond, R demon, R/2 q, R/2 q_link.
 static uint32_t meas_rows(uint32_t R) { return 2 * R + 2 * (R / 2); }
 
+// Bytes of a context's state allocation: spins and demons twice (ping-pong),
+// J, the measurement counts, each rounded up to 256 bytes.
 static size_t state_bytes(const msa_geom *geom) {
   const size_t V = (size_t)geom->L * geom->L * geom->L;
   auto up = [](size_t n) { return (n + 255) & ~(size_t)255; };

@@ -54,6 +54,23 @@ def test_microcanonical_steps(pkg, orc, L, R, nc):
     dev.close()
 
 
+@pytest.mark.parametrize("L,R,nc", [(4, 1, 1), (6, 3, 1), (16, 4, 2), (40, 5, 6), (160, 2, 1)])
+def test_measures_every_shape(pkg, orc, L, R, nc):
+    """The one-pass observables on random words: the smallest lattice, odd
+    replica counts (a replica without a pair), two pairs, six demon layers and
+    the L = 160 headline lattice (blocks of the pass cover a ragged last
+    group of rows)."""
+    print("case L=%d R=%d nc=%d" % (L, R, nc), flush=True)
+    o = random_state(L, R, nc, 7000 + L + R + nc)
+    dev = pkg.Lattice(L, R, nc)
+    dev.upload(o.J, o.S, o.D0, o.D1)
+    assert_same_measures(dev, orc, o)
+    m = dev.measure()
+    assert m["unsat"].shape == (R, 64) and m["demon"].shape == (R, 64)
+    assert ("q" in m) == (R >= 2) and (R < 2 or m["q"].shape == (R // 2, 64))
+    dev.close()
+
+
 @pytest.mark.parametrize("L,R,nc,every,niter", [(16, 1, 1, 1, 24), (16, 2, 2, 3, 12),
                                                 (24, 2, 1, 4, 16), (8, 1, 6, 2, 6)])
 def test_annealing_with_refresh(pkg, orc, L, R, nc, every, niter):
