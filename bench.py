@@ -228,14 +228,18 @@ def served(pkg, n_req, load, log2_len, max_bytes, epoch=64):
     serve = importlib.import_module("micsa_ea_mc_amd.serve")
     tables = serve.Tables()
     burst = serve.make_requests(n_req, 1e9, seed=5151, log2_len=log2_len)
-    _, cap = serve.serve(burst, max_bytes=max_bytes, epoch=epoch, tables=tables)
+    _, cap = serve.serve(burst, max_bytes=max_bytes, epoch=epoch, tables=tables, progress=progress)
     work = [64 * r.R * r.L ** 3 * r.niter for r in burst]
     capacity = sum(work) / cap["makespan_s"]                      # updates / s
     rate = load * capacity / (sum(work) / len(work))              # requests / s
     reqs = serve.make_requests(n_req, rate, seed=5151, log2_len=log2_len)
-    _, summ = serve.serve(reqs, max_bytes=max_bytes, epoch=epoch, tables=tables)
+    _, summ = serve.serve(reqs, max_bytes=max_bytes, epoch=epoch, tables=tables, progress=progress)
     # plain: FIFO, one request alone at a time
-    tp = [plain_seconds(pkg, tables, [r], [tables.job(r, r.rid)]) for r in reqs]
+    tp = []
+    for r in reqs:
+        tp.append(plain_seconds(pkg, tables, [r], [tables.job(r, r.rid)]))
+        if len(tp) % 1000 == 0:
+            progress("plain form: %d of %d requests" % (len(tp), len(reqs)))
     finish, lat = 0.0, []
     for r, t in sorted(zip(reqs, tp), key=lambda x: x[0].arrival):
         finish = max(finish, r.arrival) + t

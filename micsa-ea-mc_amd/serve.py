@@ -109,13 +109,15 @@ def run_alone(req, tables, jobid):
 
 
 def serve(requests, max_bytes, epoch=64, block=None, tables=None, keep_state=False, clock=None,
-          workers=4):
+          workers=4, progress=None):
     """Run the stream; returns (per-request dicts, summary dict).  block=None
     picks the batch kernel's block size with tune_block on one request per
     lattice size of the stream.  The refresh tables of arrived requests are
     built by `workers` host threads (the host library releases the
     interpreter lock) while the device runs; a request is admitted, in
-    arrival order, once its tables are ready and its footprint fits."""
+    arrival order, once its tables are ready and its footprint fits.
+    progress, if given, is called with a message after every 1000th
+    finished request."""
     tables = tables or Tables()
     if block is None:
         block, _ = tune_block(tables, size_sample(sorted({r.L for r in requests})))
@@ -147,6 +149,8 @@ def serve(requests, max_bytes, epoch=64, block=None, tables=None, keep_state=Fal
         done[req.rid] = rec
         updates += 64 * req.R * req.L ** 3 * req.niter
         batch.remove(slot)
+        if progress and len(done) % 1000 == 0:
+            progress("%d of %d requests finished" % (len(done), len(reqs)))
 
     try:
         while len(done) < len(reqs):
