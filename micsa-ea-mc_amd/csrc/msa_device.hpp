@@ -354,6 +354,44 @@ __device__ __forceinline__ uint64_t metropolis_update(uint64_t s, uint64_t n0, u
 // --------------------------------------------------- per-lane counting --
 // Bit-sliced (vertical) counter: word j holds bit j of 64 independent lane
 // counts.  add() is a ripple-carry add of a 1-bit-per-lane word at weight 2^pos.
+// The value of lane t ^ s for s = 1, 2, 4, 8 (DPP moves: quad permutes,
+// and row mirror / half-mirror, t ^ 15 and t ^ 7, composed with them),
+// 16 (ds_swizzle, xor mask 16 within 32 lanes) and 32 (ds_bpermute).
+__device__ __forceinline__ uint32_t lane_xor32(uint32_t v, int s) {
+  constexpr int kXor1 = 0xB1, kXor2 = 0x4E, kXor3 = 0x1B, kMirror = 0x140, kHalfMirror = 0x141;
+  switch (s) {
+    case 1: return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, kXor1, 0xF, 0xF, false);
+    case 2: return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, kXor2, 0xF, 0xF, false);
+    case 4:
+      return (uint32_t)__builtin_amdgcn_mov_dpp(
+          __builtin_amdgcn_mov_dpp((int)v, kHalfMirror, 0xF, 0xF, false), kXor3, 0xF, 0xF, false);
+    case 8:
+      return (uint32_t)__builtin_amdgcn_mov_dpp(
+          __builtin_amdgcn_mov_dpp((int)v, kMirror, 0xF, 0xF, false), kHalfMirror, 0xF, 0xF, false);
+    case 16: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x401F);
+    default: return __shfl_xor(v, 32);
+  }
+}
+__device__ __forceinline__ uint64_t lane_xor(uint64_t v, int s) {
+  return ((uint64_t)lane_xor32((uint32_t)(v >> 32), s) << 32) | lane_xor32((uint32_t)v, s);
+}
+// Transpose the 64 x 64 bit matrix held one row per lane (bit b of lane t
+// becomes bit t of lane b): six stages, each swapping the off-diagonal
+// s x s blocks of every 2s x 2s block between lanes t and t ^ s.
+__device__ __forceinline__ uint64_t wave_transpose(uint64_t x) {
+  const uint32_t lane = __lane_id();
+  constexpr uint64_t keep[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull, 0x00FF00FF00FF00FFull,
+                                0x0F0F0F0F0F0F0F0Full, 0x3333333333333333ull, 0x5555555555555555ull};
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const int s = 32 >> k;
+    const uint64_t m = keep[k];
+    const uint64_t y = lane_xor(x, s);
+    x = (lane & (uint32_t)s) ? ((x & ~m) | ((y >> s) & m)) : ((x & m) | ((y & m) << s));
+  }
+  return x;
+}
+
 // Per bit lane, the number of set bits among six words as b0 + 2 b1 + 4 b2
 // (two full adders, a half adder and a full adder over their carries).
 __device__ __forceinline__ void sum6(const uint64_t w[6], uint64_t &b0, uint64_t &b1, uint64_t &b2) {
@@ -410,43 +448,6 @@ struct VCount {
       acc += (uint32_t)__popcll(wave_transpose(c[j])) << j;
     }
     return acc;
-  }
-  // The value of lane t ^ s for s = 1, 2, 4, 8 (DPP moves: quad permutes,
-  // and row mirror / half-mirror, t ^ 15 and t ^ 7, composed with them),
-  // 16 (ds_swizzle, xor mask 16 within 32 lanes) and 32 (ds_bpermute).
-  static __device__ __forceinline__ uint32_t lane_xor32(uint32_t v, int s) {
-    constexpr int kXor1 = 0xB1, kXor2 = 0x4E, kXor3 = 0x1B, kMirror = 0x140, kHalfMirror = 0x141;
-    switch (s) {
-      case 1: return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, kXor1, 0xF, 0xF, false);
-      case 2: return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, kXor2, 0xF, 0xF, false);
-      case 4:
-        return (uint32_t)__builtin_amdgcn_mov_dpp(
-            __builtin_amdgcn_mov_dpp((int)v, kHalfMirror, 0xF, 0xF, false), kXor3, 0xF, 0xF, false);
-      case 8:
-        return (uint32_t)__builtin_amdgcn_mov_dpp(
-            __builtin_amdgcn_mov_dpp((int)v, kMirror, 0xF, 0xF, false), kHalfMirror, 0xF, 0xF, false);
-      case 16: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x401F);
-      default: return __shfl_xor(v, 32);
-    }
-  }
-  static __device__ __forceinline__ uint64_t lane_xor(uint64_t v, int s) {
-    return ((uint64_t)lane_xor32((uint32_t)(v >> 32), s) << 32) | lane_xor32((uint32_t)v, s);
-  }
-  // Transpose the 64 x 64 bit matrix held one row per lane (bit b of lane t
-  // becomes bit t of lane b): six stages, each swapping the off-diagonal
-  // s x s blocks of every 2s x 2s block between lanes t and t ^ s.
-  static __device__ __forceinline__ uint64_t wave_transpose(uint64_t x) {
-    const uint32_t lane = __lane_id();
-    constexpr uint64_t keep[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull, 0x00FF00FF00FF00FFull,
-                                  0x0F0F0F0F0F0F0F0Full, 0x3333333333333333ull, 0x5555555555555555ull};
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-      const int s = 32 >> k;
-      const uint64_t m = keep[k];
-      const uint64_t y = lane_xor(x, s);
-      x = (lane & (uint32_t)s) ? ((x & ~m) | ((y >> s) & m)) : ((x & m) | ((y & m) << s));
-    }
-    return x;
   }
 };
 

@@ -1495,18 +1495,18 @@ extern "C" int msa_measure(msa_ctx *c, int32_t *unsat, int32_t *demon, int32_t *
   a.steps = groups / fill < 1 ? 1 : (groups / fill > 16 ? 16 : groups / fill);
   const uint32_t nb = (groups + a.steps - 1) / a.steps;
   const uint32_t threads = (a.nr * g.Lh + 63) / 64 * 64;   // whole wavefronts
-  std::vector<int32_t> host(64 * (size_t)nrow);
-  HIPCHK(hipMemsetAsync(c->meas, 0, host.size() * sizeof(int32_t), c->stream));
+  int32_t host[64 * (2 * 16 + 2 * 8)];   // R <= 16 (check_geom)
+  const size_t nword = 64 * (size_t)nrow;
+  HIPCHK(hipMemsetAsync(c->meas, 0, nword * sizeof(int32_t), c->stream));
   hipLaunchKernelGGL(k_measure, dim3(nb, njob), dim3(threads), 0, c->stream, a);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(host.data(), c->meas, host.size() * sizeof(int32_t), hipMemcpyDeviceToHost,
-                        c->stream));
+  HIPCHK(hipMemcpyAsync(host, c->meas, nword * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   const size_t nR = 64 * (size_t)g.R, nP = 64 * (size_t)(g.R / 2);
-  if (unsat) memcpy(unsat, host.data(), nR * sizeof(int32_t));
-  if (demon) memcpy(demon, host.data() + nR, nR * sizeof(int32_t));
-  if (q && nP) memcpy(q, host.data() + 2 * nR, nP * sizeof(int32_t));
-  if (qlink && nP) memcpy(qlink, host.data() + 2 * nR + nP, nP * sizeof(int32_t));
+  if (unsat) memcpy(unsat, host, nR * sizeof(int32_t));
+  if (demon) memcpy(demon, host + nR, nR * sizeof(int32_t));
+  if (q && nP) memcpy(q, host + 2 * nR, nP * sizeof(int32_t));
+  if (qlink && nP) memcpy(qlink, host + 2 * nR + nP, nP * sizeof(int32_t));
   return MSA_OK;
 }
 

@@ -15,7 +15,8 @@
 //   k_walk_gen   one wavefront per (chunk, replica), lane = sample lane: each
 //                lane runs its stream over the chunk (the last 64 lagged-sum
 //                values in registers), and the 64 lanes' words are transposed
-//                with ballots into the lane layout (one word per site) and
+//                across the wavefront (DPP and swizzle stages) into the lane
+//                layout (one word per site) and
 //                stored at the sites of the reference's multispin word
 //                (packing_u placement, ini.c:301-345).
 // Lanes without a stream get zero demons.
@@ -252,15 +253,8 @@ __global__ void __launch_bounds__(64) k_walk_gen(GenArgs a) {
         d1 |= (dv >> 1) << bit;
       }
     }
-    uint64_t m0 = 0, m1 = 0;
-#pragma unroll 8
-    for (int k = 0; k < 64; ++k) {
-      const uint64_t b0 = __ballot((d0 >> k) & 1), b1 = __ballot((d1 >> k) & 1);
-      if ((int)t == k) {
-        m0 = b0;
-        m1 = b1;
-      }
-    }
+    // word for site t of the 64: bit s = lane s's bit t (all 64 lanes active)
+    const uint64_t m0 = msa::wave_transpose(d0), m1 = msa::wave_transpose(d1);
     const uint64_t iw = (uint64_t)j * a.W + w;
     const uint32_t col = (uint32_t)(iw / (HALF * g.nc));
     const uint32_t layer = (uint32_t)((iw / HALF) % g.nc);
