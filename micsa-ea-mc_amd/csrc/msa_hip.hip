@@ -732,9 +732,12 @@ struct MeasArgs {
 // demon energy is Σ D0 + 2 D1 over both colours and all layers
 // (Measure_Energy_Clerics, med.c:55-83).  blockIdx.y = 2p + k: replicas 2p
 // and 2p + 1, k = 0 the spin observables (unsatisfied bonds of each, q and
-// q_link of the pair), k = 1 the demon energies.  Counts accumulate in
-// bit-sliced counters, are transposed to lanes across the wavefront, summed
-// over the block in LDS and added to the output once per block.
+// q_link of the pair), k = 1 the demon energies.  The six bond words of a
+// site are summed per bit lane by an adder tree, the sums accumulate in
+// bit-sliced counters, which are transposed to lanes across the wavefront,
+// summed over the block in LDS and added to the output once per block.  The
+// block is nr rows of L/2 threads rounded up to whole wavefronts (the
+// transpose reads every lane; the extra lanes count nothing).
 __global__ void __launch_bounds__(512) k_measure(MeasArgs a) {
   __shared__ uint32_t bsum[4][64];
   const Geo g = a.g;
