@@ -68,6 +68,12 @@ def test_measures_every_shape(pkg, orc, L, R, nc):
     m = dev.measure()
     assert m["unsat"].shape == (R, 64) and m["demon"].shape == (R, 64)
     assert ("q" in m) == (R >= 2) and (R < 2 or m["q"].shape == (R // 2, 64))
+    # any output may be NULL: the demon counts alone
+    import ctypes
+    demon = np.zeros((R, 64), np.int32)
+    assert dev.lib.msa_measure(dev.ctx, None, demon.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                               None, None) == 0
+    assert np.array_equal(demon, m["demon"])
     dev.close()
 
 
