@@ -161,3 +161,18 @@ def test_library_exports_every_declared_symbol(pkg, lib, header):
     missing = [s for s in _declared(header) if s not in exported]
     assert not missing, missing
     C.CDLL(path)   # loads without a GPU
+
+
+@pytest.mark.parametrize("L,R,nc", [(4, 1, 1), (16, 3, 2), (160, 2, 1), (64, 16, 6)])
+def test_footprint_is_the_state_allocation(pkg, L, R, nc):
+    """msa_footprint_bytes (what serving admits by) is the one device
+    allocation of a lattice: spins and demons twice (ping-pong), J, and the
+    measurement counts (R unsat, R demon, R/2 q, R/2 q_link rows of 64 int32),
+    each rounded up to 256 bytes.  Host arithmetic only, no GPU."""
+    V = L ** 3
+    up = lambda n: (n + 255) // 256 * 256  # noqa: E731
+    want = (2 * up(8 * V * R) + up(8 * V * 3) + 2 * up(16 * V * R * nc)
+            + up(4 * 64 * (2 * R + 2 * (R // 2))))
+    assert pkg.footprint_bytes(L, R, nc) == want
+    assert pkg.footprint_bytes(L + 1, R, nc) == -1          # odd L is rejected
+    assert pkg.footprint_bytes(L, 17, nc) == -1             # at most 16 replicas
