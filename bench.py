@@ -10,7 +10,11 @@ timed region covers exactly --steps sweeps and the measurements of the
 integer energies, demon energies, q and q_link every 1024 sweeps (the
 configuration's cadence), bracketed by stream syncs and timed with the host
 clock (the sweeps alone, timed with HIP events on the library's stream, are
-reported beside it).
+reported beside it).  The configuration's work mix is one refresh sweep and
+one measurement per 1024 sweeps: when the window holds fewer of them than
+its length calls for (a window shorter than 1024 sweeps holds none), the
+extra cost of a refresh block and of a measurement, timed separately after
+the window, is added to the window's time for the share it lacks.
 
 The sweep kernel and its tile are first autotuned on this lattice (every
 candidate gives bit-identical results; --no-autotune keeps the default).
@@ -24,7 +28,8 @@ sweeps that all use it, against the MI355X HBM peak of 8 TB/s.  `traffic`
 (HBM bytes per launch from counters) is read from the profile summary in
 profiles/ when one exists for the same kernel and workload.
 `cpu_baseline` times the oracle's word-level CPU sweep (tests/oracle_lib.py)
-on a bounded sample of the same lattice on this host.
+on a bounded sample of the same lattice on this host, with one refresh sweep
+and one measurement folded in at the same cadence.
 """
 import argparse
 import json
@@ -88,11 +93,29 @@ def alg_bytes(V, R, nsweeps, nrefresh):
     return (nsweeps - nrefresh) * (48 * R + 24) * V + nrefresh * (32 * R + 24) * V
 
 
-def cpu_baseline(L, R, nc, budget_s, threads):
-    """Oracle sweeps on this host's cores for about budget_s seconds."""
+def host_cpu():
+    """nproc, the cores this process may run on, and the CPU model string."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)), "model": model}
+
+
+def cpu_baseline(L, R, nc, budget_s, threads, every, measure_every):
+    """The oracle on this host's cores: refresh-free sweeps for about budget_s
+    seconds, then one refresh sweep and one measurement (energies, demon
+    energies, q, q_link), folded in at the configuration's cadence (one of
+    each per `every` / `measure_every` sweeps)."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib
+    from sched_util import oracle_schedule
     orc = oracle_lib.load()
     o = oracle_lib.Lattice(L, R, nc)
     orc.init_philox(o, 0x5EED)
@@ -107,10 +130,35 @@ def cpu_baseline(L, R, nc, budget_s, threads):
         dt = time.perf_counter() - t0
         if dt >= budget_s:
             break
-    return {"value": 64.0 * R * L ** 3 * n / dt / 1e12, "unit": "spin updates/ps",
+    t_sweep = dt / n
+    ctr, keys, _, luts = oracle_schedule(orc, R, [every], [1.0 / 1.8])
+    r0 = time.perf_counter()
+    orc.iteration(o, ctr[0], keys, luts[0], nthreads=threads)
+    t_ref = time.perf_counter() - r0
+    m0 = time.perf_counter()
+    orc.energy(o)
+    orc.demon_energy(o)
+    for p in range(R // 2):
+        orc.overlap(o, 2 * p, 2 * p + 1)
+    t_meas = time.perf_counter() - m0
+    per_sweep = t_sweep + ((t_ref - t_sweep) / every if every > 0 else 0.0) \
+        + (t_meas / measure_every if measure_every > 0 else 0.0)
+    cpu = host_cpu()
+    return {"value": 64.0 * R * L ** 3 / per_sweep / 1e12, "unit": "spin updates/ps",
             "cores": threads, "kind": "port",
-            "sample": "%d sweeps of the L=%d R=%d lattice (no refresh), oracle word-level sweep, "
-                      "%.1f s" % (n, L, R, dt)}
+            "sample": "oracle word-level sweep of the L=%d R=%d lattice on %d threads: %d "
+                      "refresh-free sweeps in %.1f s (%.1f ms each), one refresh sweep %.1f ms and "
+                      "one measurement %.1f ms folded in per %d and %d sweeps"
+                      % (L, R, threads, n, dt, t_sweep * 1e3, t_ref * 1e3, t_meas * 1e3, every,
+                         measure_every),
+            "nproc": cpu["nproc"], "cpu_affinity": cpu["affinity"], "cpu_model": cpu["model"]}
+
+
+def nref_window(t0, steps, every):
+    """Refreshes among iterations t0 .. t0+steps-1 at a uniform cadence."""
+    if every <= 0:
+        return 0
+    return sum(1 for t in range(t0, t0 + steps) if t % every == 0)
 
 
 def progress(msg):
@@ -271,7 +319,9 @@ def main():
     ap.add_argument("--measure-every", type=int, default=1024,
                     help="measure energies and overlaps every this many sweeps inside the timed region (0: never)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
-    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-threads", type=int, default=min(16, len(os.sched_getaffinity(0))),
+                    help="oracle threads (default: the cores this process may use, at most 16, "
+                         "the host share of one GPU)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batched", type=int, default=512, help="lattices in the batched test (0: skip)")
     ap.add_argument("--batched-len-div", type=int, default=16,
@@ -308,7 +358,10 @@ def main():
     tiling = lat.tiling()
     total = args.warmup + args.steps
     sched = pkg.schedule(1.8, 0.6, nsched, every)
-    ctr, keys, luts, idx = pkg.schedule_tables(sched, total, R, [3003 + r for r in range(R)], 0)
+    # tables reach two refreshes past the timed window: the refresh block
+    # timed after it (below) starts at the first refresh after the window
+    ntab = total + 2 * every if every > 0 else total
+    ctr, keys, luts, idx = pkg.schedule_tables(sched, ntab, R, [3003 + r for r in range(R)], 0)
     lat.set_luts(luts)
     lat.set_refresh(ctr, keys, idx)
 
@@ -331,9 +384,29 @@ def main():
     lat.sync()
     wall = time.perf_counter() - w0
     m = lat.measure()
+    # the configuration's work mix has one refresh per `every` and one
+    # measurement per `me` sweeps; a window shorter than that holds a share
+    # of them only by chance, so their cost is timed here and the share the
+    # window lacks is added to its time (value = updates / (wall + missing))
+    t_after = args.warmup + args.steps
+    t_ref_extra = 0.0
+    if every > 0:
+        tb = (t_after // every + 1) * every          # first refresh after the window
+        ms_ref = lat.run_timed(tb, every, every)     # `every` sweeps, the first one refreshing
+        ms_plain = lat.run_timed(tb + every, every, 0)
+        t_ref_extra = max(0.0, (ms_ref - ms_plain) * 1e-3)
+    t_meas = 0.0
+    if me > 0:
+        q0 = time.perf_counter()
+        for _ in range(3):
+            lat.measure()
+        t_meas = (time.perf_counter() - q0) / 3
+    miss_ref = (args.steps / every - nref_window(args.warmup, args.steps, every)) if every > 0 else 0.0
+    miss_meas = (args.steps / me - nmeas) if me > 0 else 0.0
+    missing = miss_ref * t_ref_extra + miss_meas * t_meas
     # the dominant kernel alone: sweeps without a refresh (an even count, so
     # that the sweep-pair kernel runs alone)
-    ksweeps = max(2, min(args.steps, 1024) & ~1)
+    ksweeps = max(2, min(max(args.steps, 512), 1024) & ~1)
     ms_k = lat.run_timed(args.warmup + args.steps, ksweeps, 0)
     # the state across the boundary: host arrays in and out (PCIe), once
     p0 = time.perf_counter()
@@ -376,9 +449,8 @@ def main():
         copy = {"bytes": args.copy_probe_mb << 20,
                 "GBs": round(2.0 * (args.copy_probe_mb << 20) / (ms_c * 1e-3) / 1e9, 1)}
 
-    t0 = args.warmup
-    nref = sum(1 for t in range(t0, t0 + args.steps) if every > 0 and t % every == 0)
-    sec = wall
+    nref = nref_window(args.warmup, args.steps, every)
+    sec = wall + missing
     updates = 64.0 * R * V * args.steps
     value = updates / sec / 1e12
     per_sweep = {0: 2.0, 1: 1.0, 2: 1.0, 3: 0.5}[tiling[0]] * nc
@@ -387,6 +459,10 @@ def main():
     avg_s = ms_k * 1e-3 / launches_k
     achieved = kbytes / avg_s / 1e9
     kname = kernel_name(tiling[0], R)
+    traffic = read_traffic(kname, L, R, tiling)
+    traffic_src = ("recorded by rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes over this "
+                   "kernel, lattice and tile (profiles/traffic_*.txt), not measured in this run"
+                   if traffic is not None else None)
     out = {
         "metric": "spin_updates_per_ps",
         "value": round(value, 4),
@@ -394,7 +470,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(wall * 1e3 / args.steps, 5),
+        "ms_per_step": round(sec * 1e3 / args.steps, 5),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -417,7 +493,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": read_traffic(kname, L, R, tiling),
+            "traffic": traffic,
+            "traffic_source": traffic_src,
             "kernel": kname,
             "avg_launch_us": round(avg_s * 1e6, 3),
             "launches_timed": launches_k,
@@ -425,6 +502,14 @@ def main():
             "timed_region_alg_GBs": round(alg_bytes(V, R, args.steps, nref) / (ms_sweeps * 1e-3) / 1e9, 1),
         },
         "wall_s": round(wall, 4),
+        "work_mix": {"note": "value = updates / (wall of the timed window + the cost of the "
+                             "refreshes and measurements the window lacks at the configuration's "
+                             "cadence, timed separately)",
+                     "refresh_every": every, "measure_every": me,
+                     "refresh_extra_ms": round(t_ref_extra * 1e3, 4),
+                     "measurement_ms": round(t_meas * 1e3, 4),
+                     "refreshes_added": round(miss_ref, 5), "measurements_added": round(miss_meas, 5),
+                     "added_ms": round(missing * 1e3, 5)},
         "sweeps_only": {"spin_updates_per_ps": round(updates / (ms_sweeps * 1e-3) / 1e12, 4),
                         "ms_per_sweep": round(ms_sweeps / args.steps, 5),
                         "note": "HIP events around the sweeps of the timed region, measurements excluded"},
@@ -451,7 +536,7 @@ def main():
         out["served"] = served(pkg, args.served, args.served_load, (6, 11), 8 << 30)
     if not args.no_cpu:
         progress("CPU baseline: %.0f s" % args.cpu_budget)
-        out["cpu_baseline"] = cpu_baseline(L, R, nc, args.cpu_budget, args.cpu_threads)
+        out["cpu_baseline"] = cpu_baseline(L, R, nc, args.cpu_budget, args.cpu_threads, every, me)
     print(json.dumps(out))
 
 

@@ -102,8 +102,12 @@ double  msa_sched_beta(const msa_sched *s, int64_t k);
 /* Build the refresh tables for refreshes 0..nref-1: counters ctr[k*R + r]
  * from per-replica generators seeded with seed_MC[r] (Init_Random,
  * ini.c:49-51) filled as _fill_bits at iteration t_k, parity 0; keys[r];
- * LUT pairs luts[2k], luts[2k+1]; lut_idx[k] = k.  Tables are built in
- * parallel over k for the LUTs (OpenMP), sequentially for the counters. */
+ * LUT pairs luts[2n], luts[2n+1] for the distinct inverse temperatures of
+ * the refreshes in order, and lut_idx[k] = the pair of refresh k (refreshes
+ * at equal beta share one pair; luts must hold room for nref pairs).
+ * Returns the number of pairs written, or -1 on bad arguments.  The LUTs
+ * are built in parallel (OpenMP), the counters sequentially.  Past the end
+ * of the anneal (t > nsweeps) the temperature stays at T1. */
 int msa_sched_tables(const msa_sched *s, int64_t nref, int R, const uint64_t *seed_MC,
                      uint32_t jobid, msa_u32x4 *ctr, uint32_t *keys, msa_lut *luts,
                      int32_t *lut_idx);

@@ -124,8 +124,10 @@ __device__ __forceinline__ LutView lut_view(const FineLut *t) {
 // Byte n with umbral[n-1] <= R < umbral[n], R = (rm:rl): the bisection of
 // main.cu:615-630 started from the bracket [pre[b], pre[b+1]] of R's bucket
 // b, which always contains the answer (the thresholds are non-decreasing,
-// checked on upload).  Same byte for every R as the reference's search
-// (checked against it on every interval endpoint).
+// checked on upload).  Same byte for every R as the reference's search: the
+// table construction and this search, restated on the host, agree with the
+// reference bisection on every interval endpoint and bucket edge, and the
+// device's refresh words equal the oracle's in every refresh parity test.
 __device__ __forceinline__ uint32_t bisect(const LutView &t, uint32_t rm, uint32_t rl) {
   auto below = [&](uint32_t n) {
     const uint2 u = t.umb[n];

@@ -161,14 +161,14 @@ __global__ void __launch_bounds__(1024) k_halfsweep(SweepArgs a) {
   }
 }
 
-// Search tables of `ntab` heat-bath tables (msa::FineLut): thresholds copied,
-// presearch entry b = the byte of the first R of bucket b, found by binary
-// search for the smallest n with R < umbral[n] (255 when none), and the
-// sentinel 255 after the last bucket.
-__global__ void __launch_bounds__(256) k_fine_luts(const msa_lut *luts, uint32_t ntab,
-                                                   msa::FineLut *out) {
+// Search tables of heat-bath tables first .. ntab-1 (msa::FineLut), table
+// first + blockIdx.y: thresholds copied, presearch entry b = the byte of the
+// first R of bucket b, found by binary search for the smallest n with
+// R < umbral[n] (255 when none), and the sentinel 255 after the last bucket.
+__global__ void __launch_bounds__(256) k_fine_luts(const msa_lut *luts, uint32_t first,
+                                                   uint32_t ntab, msa::FineLut *out) {
   const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t tab = blockIdx.y;
+  const uint32_t tab = first + blockIdx.y;
   constexpr uint32_t nb = 1u << msa::kPreBits;
   if (b >= nb + 16 || tab >= ntab) return;
   if (b >= nb) {
@@ -226,8 +226,13 @@ int msa_build_fine_luts(hipStream_t st, int32_t nlut, const msa_lut *host, msa::
   if (hipMemcpyAsync(raw, host, sizeof(msa_lut) * ntab, hipMemcpyHostToDevice, st) != hipSuccess)
     rc = msa_fail(MSA_ERR_HIP, "LUT upload");
   if (!rc) {
-    hipLaunchKernelGGL(k_fine_luts, dim3((1u << msa::kPreBits) / 256 + 1, (uint32_t)ntab), dim3(256), 0, st,
-                       raw, (uint32_t)ntab, *out);
+    // gridDim.y is at most 65535: launches of up to that many tables
+    constexpr uint32_t kMaxY = 65535;
+    for (size_t first = 0; first < ntab; first += kMaxY) {
+      const uint32_t n = (uint32_t)(ntab - first < kMaxY ? ntab - first : kMaxY);
+      hipLaunchKernelGGL(k_fine_luts, dim3((1u << msa::kPreBits) / 256 + 1, n), dim3(256), 0, st,
+                         raw, (uint32_t)first, (uint32_t)ntab, *out);
+    }
     if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
       rc = msa_fail(MSA_ERR_HIP, "search table construction");
   }

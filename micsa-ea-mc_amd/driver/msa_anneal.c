@@ -717,8 +717,11 @@ int main(int argc, char **argv)
     msa_u32x4 *ctr = (msa_u32x4 *)xmalloc(sizeof(msa_u32x4) * (size_t)(nref * R), "refresh counters");
     msa_lut *luts = (msa_lut *)xmalloc(sizeof(msa_lut) * 2 * (size_t)nref, "refresh tables");
     int32_t *lut_idx = (int32_t *)xmalloc(sizeof(int32_t) * (size_t)nref, "refresh indices");
-    msa_sched_tables_resume(&sch, k_start, nref, R, pr, xo, (uint32_t)sample, ctr, luts, lut_idx);
-    CHK(msa_set_luts(ctx, (int32_t)nref, luts));
+    /* one LUT pair per distinct beta: a fixed-beta run builds one */
+    const int npair = msa_sched_tables_resume(&sch, k_start, nref, R, pr, xo, (uint32_t)sample,
+                                              ctr, luts, lut_idx);
+    if (npair < 1) die("Problems building the refresh tables (nref=%lld)", (long long)nref);
+    CHK(msa_set_luts(ctx, npair, luts));
     CHK(msa_set_refresh(ctx, (int32_t)nref, ctr, keys, lut_idx));
     free(luts);
     free(ctr);

@@ -8,6 +8,7 @@
  */
 #include <math.h>
 #include <quadmath.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include "msa_host.h"
@@ -95,8 +96,9 @@ int64_t msa_sched_time(const msa_sched *s, int64_t k)
 double msa_sched_beta(const msa_sched *s, int64_t k)
 {
   if (s->T0 <= 0) return s->beta_fixed;
-  const double t = (double)msa_sched_time(s, k);
-  const double T = s->T0 + (s->T1 - s->T0) * t / (double)s->nsweeps;
+  int64_t tk = msa_sched_time(s, k);
+  if (tk > s->nsweeps) tk = s->nsweeps;   /* past the end of the anneal T stays at T1 */
+  const double T = s->T0 + (s->T1 - s->T0) * (double)tk / (double)s->nsweeps;
   return 1.0 / T;
 }
 
@@ -105,6 +107,12 @@ int msa_sched_tables_resume(const msa_sched *s, int64_t k0, int64_t nref, int R,
                             int32_t *lut_idx)
 {
   if (R < 1 || R > MSA_MAXR || nref < 0 || k0 < 0) return -1;
+  /* refreshes at the same inverse temperature share one LUT pair (a schedule
+   * is monotone, so equal betas are consecutive): a fixed-beta run, or the
+   * part of a run past the end of its anneal, builds one pair */
+  double *beta = (double *)malloc(sizeof(double) * (size_t)(nref > 0 ? nref : 1));
+  if (!beta) return -1;
+  int64_t npair = 0;
   for (int64_t k = 0; k < nref; k++) {
     const uint64_t t = (uint64_t)msa_sched_time(s, k);
     for (int r = 0; r < R; r++) {
@@ -113,11 +121,14 @@ int msa_sched_tables_resume(const msa_sched *s, int64_t k0, int64_t nref, int R,
       else
         memset(&ctr[k * R + r], 0, sizeof(msa_u32x4));
     }
-    lut_idx[k] = (int32_t)k;
+    const double b = msa_sched_beta(s, k);
+    if (npair == 0 || b != beta[npair - 1]) beta[npair++] = b;
+    lut_idx[k] = (int32_t)(npair - 1);
   }
 #pragma omp parallel for schedule(dynamic, 16)
-  for (int64_t k = 0; k < nref; k++) msa_lut_demon(msa_sched_beta(s, k), &luts[2 * k], &luts[2 * k + 1]);
-  return 0;
+  for (int64_t n = 0; n < npair; n++) msa_lut_demon(beta[n], &luts[2 * n], &luts[2 * n + 1]);
+  free(beta);
+  return (int)npair;
 }
 
 int msa_sched_tables_from(const msa_sched *s, int64_t nref, int R, msa_prng *pr, msa_xo256 *xo,

@@ -447,15 +447,16 @@ def schedule_tables_from(sched, niter, gens, jobid=0):
     ctr = np.zeros((nref, R, 4), np.uint32)
     luts = np.zeros(2 * nref * LUT_BYTES, np.uint8)
     idx = np.zeros(nref, np.int32)
-    rc = H.msa_sched_tables_from(C.byref(sched), nref, R, pr, xo, jobid, _p(ctr, u32p),
-                                 luts.ctypes.data_as(C.c_void_p), _p(idx, i32p))
-    if rc:
+    npair = H.msa_sched_tables_from(C.byref(sched), nref, R, pr, xo, jobid, _p(ctr, u32p),
+                                    luts.ctypes.data_as(C.c_void_p), _p(idx, i32p))
+    if npair < 0:
         raise MsaError("schedule tables failed")
-    return ctr, keys.copy(), luts, idx
+    return ctr, keys.copy(), luts[:2 * npair * LUT_BYTES], idx
 
 
 def schedule_tables(sched, niter, R, seed_mc, jobid=0):
-    """Counters (nref,R,4) u32, keys (R,) u32, LUT bytes (2*nref tables), lut_idx (nref,)."""
+    """Counters (nref,R,4) u32, keys (R,) u32, LUT bytes (one pair of tables per
+    distinct beta), lut_idx (nref,) = the pair of each refresh."""
     H = host()
     nref = H.msa_sched_count(C.byref(sched), niter)
     ctr = np.zeros((nref, R, 4), np.uint32)
@@ -463,11 +464,11 @@ def schedule_tables(sched, niter, R, seed_mc, jobid=0):
     luts = np.zeros(2 * nref * LUT_BYTES, np.uint8)
     idx = np.zeros(nref, np.int32)
     seeds = np.ascontiguousarray(seed_mc, np.uint64)
-    rc = H.msa_sched_tables(C.byref(sched), nref, R, _p(seeds), jobid, _p(ctr, u32p),
-                            _p(keys, u32p), luts.ctypes.data_as(C.c_void_p), _p(idx, i32p))
-    if rc:
+    npair = H.msa_sched_tables(C.byref(sched), nref, R, _p(seeds), jobid, _p(ctr, u32p),
+                               _p(keys, u32p), luts.ctypes.data_as(C.c_void_p), _p(idx, i32p))
+    if npair < 0:
         raise MsaError("schedule tables failed")
-    return ctr, keys, luts, idx
+    return ctr, keys, luts[:2 * npair * LUT_BYTES], idx
 
 
 class Batch:

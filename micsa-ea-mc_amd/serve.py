@@ -59,12 +59,24 @@ class Tables:
         self.nlut = int(round(T_HI / TGRID)) - self.nlo + 1
         self.luts = msa.lut_grid(TGRID, self.nlo, self.nlut)
 
-    def job(self, req, jobid):
+    def refresh_tables(self, req, jobid):
+        """Counters (nref, R, 4), keys (R,) and grid indices (nref,) of a request:
+        seed_MC[r] = seed + r, jobid = request id mod 2^14, refresh every sweep."""
         sched = msa.schedule(req.T0, req.T1, req.niter, 1)
-        ctr, keys, idx = msa.schedule_tables_grid(sched, req.niter, req.R,
-                                                  [req.seed + r for r in range(req.R)],
-                                                  jobid % 16384, TGRID, self.nlo, self.nlut)
+        return msa.schedule_tables_grid(sched, req.niter, req.R,
+                                        [req.seed + r for r in range(req.R)],
+                                        jobid % 16384, TGRID, self.nlo, self.nlut)
+
+    def job(self, req, jobid):
+        ctr, keys, idx = self.refresh_tables(req, jobid)
         return msa.Batch.make_job(req.L, req.R, req.seed, req.niter, 1, ctr, keys, idx)
+
+    def subset(self, idx):
+        """The LUT pairs that grid indices `idx` use, as a table of their own,
+        and the indices into it."""
+        used, remap = np.unique(np.asarray(idx), return_inverse=True)
+        raw = np.frombuffer(bytes(self.luts), np.uint8).reshape(self.nlut, 2 * msa.LUT_BYTES)
+        return np.ascontiguousarray(raw[used]).reshape(-1), remap.astype(np.int32)
 
 
 BLOCKS = (256, 512, 1024)
@@ -98,14 +110,22 @@ def size_sample(sizes=(16, 32, 48, 64, 96, 128), niter=64, T0=2.0, T1=0.7):
 
 
 def run_alone(req, tables, jobid):
-    """The request by itself (a batch holding only it): the reference result."""
-    b = msa.Batch(tables.luts)
-    slot = b.add(tables.job(req, jobid))
-    b.run(req.niter)
-    b.sync()
-    out = b.download(slot), b.measure(slot)
-    b.close()
-    return out
+    """The request by itself on the plain path: one lattice through msa_run
+    (the single-lattice kernels, not the batch kernel), state built by
+    msa_init_philox(seed), the same counters, keys and grid LUT pairs.  Returns
+    ((spins, d0, d1), per-lane observables): the result the served form must
+    reproduce bit for bit."""
+    ctr, keys, idx = tables.refresh_tables(req, jobid)
+    luts, sub_idx = tables.subset(idx)
+    lat = msa.Lattice(req.L, req.R, 1)
+    try:
+        lat.init_philox(req.seed)
+        lat.set_luts(luts)
+        lat.set_refresh(ctr, keys, sub_idx)
+        lat.run(0, req.niter, 1)
+        return lat.download(), lat.measure()
+    finally:
+        lat.close()
 
 
 def serve(requests, max_bytes, epoch=64, block=None, tables=None, keep_state=False, clock=None,

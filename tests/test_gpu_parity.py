@@ -330,3 +330,41 @@ def test_L64_R2_configuration_start(pkg, orc):
         assert_same_measures(dev, orc, o)
     assert_same_state(dev, o, "after %d sweeps" % nsweeps)
     dev.close()
+
+
+@pytest.mark.parametrize("tile", ["5x40", "autotuned"])
+def test_L160_roofline_lattice_matches_oracle(pkg, orc, tile):
+    """The L = 160, R = 2, nc = 1 lattice of the roofline configuration against
+    the oracle, on the sweep-pair kernel with the 5 x 40 tile and on the kernel
+    and tile msa_autotune picks for this geometry (the choice bench.py times).
+    Iterations 1..4 are refresh-free (two sweep-pair launches); iteration 5
+    refreshes (the two-pass refresh kernel, Philox + LUT demons); 6 and 7 are
+    one more pair.  Spins, demons and every per-lane count (unsatisfied bonds,
+    demon energy, q, q_link) are compared bit-exactly after each run."""
+    L, R, nc, every = 160, 2, 1, 5
+    o = random_state(L, R, nc, 160160)
+    times = [0, 5]
+    ctr, keys, raw, luts = oracle_schedule(orc, R, times, [1 / 1.8, 1 / 1.79], jobid=7)
+    dev = pkg.Lattice(L, R, nc)
+    if tile == "5x40":
+        dev.set_tiling(3, 5, 40)
+    else:
+        dev.autotune(16)
+    print("case L=160 R=2 tiling %s -> %s" % (tile, dev.tiling()), flush=True)
+    if tile == "5x40":
+        assert dev.tiling() == (3, 5, 40)
+    dev.upload(o.J, o.S, o.D0, o.D1)
+    dev.set_luts(raw)
+    dev.set_refresh(ctr, keys, np.arange(len(times), dtype=np.int32))
+    t = 1
+    for n in (4, 3):
+        dev.run(t, n, every)
+        for u in range(t, t + n):
+            if u % every == 0:
+                orc.iteration(o, ctr[u // every], keys, luts[u // every], nthreads=16)
+            else:
+                orc.iteration(o, nthreads=16)
+        t += n
+        assert_same_state(dev, o, "after t=%d" % t)
+        assert_same_measures(dev, orc, o)
+    dev.close()

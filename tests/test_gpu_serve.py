@@ -1,15 +1,67 @@
-"""Served form: requests admitted into a running batch as memory frees up;
-every result must equal the same request run alone (bit-exact)."""
+"""Served form: requests admitted into a running batch as memory frees up.
+
+Every served result must equal the same request run alone on the plain path
+(one lattice through msa_run: the single-lattice kernels, not the batch
+kernel, with the same counters, keys and grid LUT pairs), bit for bit, and
+the small lattices must also equal the oracle run of the request."""
+import math
+
 import numpy as np
 import pytest
+
+from oracle_lib import Lattice as OLattice
+from sched_util import oracle_schedule
 
 pytestmark = pytest.mark.gpu
 
 
-def test_serving_mix_equals_alone(pkg):
+def modules():
     import importlib
-    serve = importlib.import_module("micsa_ea_mc_amd.serve")
-    msa = importlib.import_module("micsa_ea_mc_amd.msa")
+    return (importlib.import_module("micsa_ea_mc_amd.serve"),
+            importlib.import_module("micsa_ea_mc_amd.msa"))
+
+
+def grid_betas(serve, req):
+    """Inverse temperature of every refresh of a served request: the linear
+    anneal T0 -> T1 over niter sweeps, refresh every sweep, T rounded to the
+    serving grid (msa_sched_tables_grid restated)."""
+    nlo = int(round(serve.T_LO / serve.TGRID))
+    nlut = int(round(serve.T_HI / serve.TGRID)) - nlo + 1
+    out = []
+    for t in range(req.niter):
+        beta = 1.0 / (req.T0 + (req.T1 - req.T0) * t / req.niter)
+        n = min(max(int(math.floor((1.0 / beta) / serve.TGRID + 0.5)) - nlo, 0), nlut - 1)
+        out.append(1.0 / (serve.TGRID * (nlo + n)))
+    return out
+
+
+def oracle_run(orc, serve, req):
+    """The request on the oracle: state of msa_init_philox(seed), counters
+    from the generators of seed_MC[r] = seed + r with jobid = rid mod 2^14."""
+    o = OLattice(req.L, req.R, 1)
+    orc.init_philox(o, req.seed)
+    betas = grid_betas(serve, req)
+    times = list(range(req.niter))
+    ctr, keys, _, luts = oracle_schedule(orc, req.R, times, betas, seed_mc=req.seed,
+                                         jobid=req.rid % 16384)
+    for t in times:
+        orc.iteration(o, ctr[t], keys, luts[t], nthreads=16)
+    return o
+
+
+def check_against_alone(serve, tables, reqs, done):
+    for r in reqs:
+        rec = done[r.rid]
+        assert rec["finish"] >= rec["admit"] >= r.arrival - 1e-9
+        (S, D0, D1), m = serve.run_alone(r, tables, r.rid)
+        assert np.array_equal(S, rec["state"][0]), "request %d: spins differ alone vs mixed" % r.rid
+        assert np.array_equal(D0, rec["state"][1]) and np.array_equal(D1, rec["state"][2]), \
+            "request %d: demons differ alone vs mixed" % r.rid
+        assert np.array_equal(m["energy"], rec["energy"]), "request %d: energies differ" % r.rid
+
+
+def test_serving_mix_equals_alone():
+    serve, msa = modules()
     reqs = serve.make_requests(10, rate=400.0, seed=5151, sizes=(8, 16, 24), log2_len=(5, 8))
     tables = serve.Tables()
     # room for about three of the largest lattices at once: admission must queue
@@ -24,11 +76,40 @@ def test_serving_mix_equals_alone(pkg):
     assert summary["max_bytes_in_use"] <= limit          # admission by footprint held
     assert summary["max_concurrent"] >= 2                 # requests overlapped on the device
     assert any(done[r.rid]["admit"] > r.arrival + 1e-4 for r in reqs)   # someone waited
-    for r in reqs:
-        rec = done[r.rid]
-        assert rec["finish"] >= rec["admit"] >= r.arrival - 1e-9
-        (S, D0, D1), m = serve.run_alone(r, tables, r.rid)
-        assert np.array_equal(S, rec["state"][0]), "request %d: spins differ alone vs mixed" % r.rid
-        assert np.array_equal(D0, rec["state"][1]) and np.array_equal(D1, rec["state"][2])
-        assert np.array_equal(m["energy"], rec["energy"])
+    check_against_alone(serve, tables, reqs, done)
     assert summary["latency_p99_s"] >= summary["latency_p50_s"] > 0
+
+
+def test_served_configuration_subset(orc):
+    """The served configuration's mix on 100 requests at short lengths: L
+    uniform over {16, 32, 48, 64, 96, 128}, 64 +-J samples, 2^5..2^8 sweeps
+    from T = 2.0 to a final T uniform in [0.6, 0.9], Poisson arrivals (seed
+    5151), admission by footprint under a limit of three L = 128 lattices, so
+    requests queue and overlap.  Every request's spins, demons and energies
+    equal the request run alone on the plain path; the L <= 32 requests also
+    equal the oracle."""
+    serve, msa = modules()
+    reqs = serve.make_requests(100, rate=2000.0, seed=5151, log2_len=(5, 8))
+    assert {r.L for r in reqs} == {16, 32, 48, 64, 96, 128}
+    tables = serve.Tables()
+    probe = msa.Batch(tables.luts)
+    limit = 3 * max(probe.job_bytes(tables.job(r, r.rid)) for r in reqs)
+    probe.close()
+    done, summary = serve.serve(reqs, max_bytes=limit, epoch=32, block=1024, tables=tables,
+                                keep_state=True)
+    print("served %d requests: %.3f s, max %d resident, %.2f spin updates/ps" % (
+        summary["requests"], summary["makespan_s"], summary["max_concurrent"],
+        summary["spin_updates_per_ps"]), flush=True)
+    assert summary["requests"] == len(reqs)
+    assert summary["max_bytes_in_use"] <= limit
+    assert summary["max_concurrent"] >= 3
+    check_against_alone(serve, tables, reqs, done)
+    small = [r for r in reqs if r.L <= 32]
+    assert len(small) >= 20
+    for r in small:
+        o = oracle_run(orc, serve, r)
+        rec = done[r.rid]
+        assert np.array_equal(rec["state"][0], o.S), "request %d: spins differ from the oracle" % r.rid
+        assert np.array_equal(rec["state"][1], o.D0) and np.array_equal(rec["state"][2], o.D1), \
+            "request %d: demons differ from the oracle" % r.rid
+        assert np.array_equal(rec["energy"], orc.energy(o)), "request %d: energies" % r.rid

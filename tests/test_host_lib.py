@@ -122,8 +122,30 @@ def test_schedule_tables_match_oracle(pkg, orc, every, niter):
     octr, okeys, oraw, _ = oracle_schedule(orc, R, times, betas, seed_mc=3003, jobid=9)
     assert np.array_equal(ctr, octr)
     assert np.array_equal(keys, okeys)
-    assert luts.tobytes() == oraw
-    assert list(idx) == list(range(nref))
+    # one LUT pair per distinct beta; refresh k uses pair idx[k]
+    pair = 2 * 4096
+    assert luts.size // pair == len(set(betas)) and list(idx) == sorted(idx)
+    for k in range(nref):
+        assert luts[idx[k] * pair:(idx[k] + 1) * pair].tobytes() == oraw[k * pair:(k + 1) * pair]
+    if every > 0:
+        assert list(idx) == list(range(nref))
+    else:
+        assert list(idx) == [0] * nref        # fixed beta: a single pair
+
+
+def test_schedule_holds_T1_past_its_end(pkg):
+    """Refreshes after the last sweep of the anneal (t > nsweeps) stay at T1
+    instead of extrapolating the line to lower and then negative T; they
+    share the LUT pair of T1."""
+    H = pkg.host()
+    sch = pkg.schedule(2.0, 0.7, 256, 4)
+    beta_end = H.msa_sched_beta(C.byref(sch), 64)          # t = 256
+    assert abs(1.0 / beta_end - 0.7) < 1e-12
+    for k in (65, 100, 120, 1000):                          # t = 260 .. 4000
+        assert H.msa_sched_beta(C.byref(sch), k) == beta_end
+    ctr, keys, luts, idx = pkg.schedule_tables(sch, 480, 1, [3003], jobid=0)
+    assert ctr.shape[0] == 120 and luts.size == 2 * 4096 * 65
+    assert list(idx[:65]) == list(range(65)) and all(i == 64 for i in idx[64:])
 
 
 def test_thermal_counters_and_continuation_match_oracle(pkg, orc):
@@ -143,7 +165,7 @@ def test_thermal_counters_and_continuation_match_oracle(pkg, orc):
     betas = [H.msa_sched_beta(C.byref(sch), k) for k in range(nref)]
     rctr, rkeys, raw, _ = oracle_schedule(orc, R, times, betas, jobid=jobid, gens=ogens)
     assert np.array_equal(ctr, rctr) and np.array_equal(keys, rkeys)
-    assert luts.tobytes() == raw
+    assert luts.tobytes() == raw and list(idx) == list(range(nref))
 
 
 def _declared(header):
