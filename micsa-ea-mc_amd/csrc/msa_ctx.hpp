@@ -28,6 +28,7 @@ struct msa_ctx {
   void *slab = nullptr;                   // the one allocation holding sp, jp, dm and meas
   uint64_t *th_rnd = nullptr;             // thermal random words [3][V/128 * R]
   uint32_t hnr = 1, hgrid = 1;            // two-pass kernel: rows per tile, persistent blocks
+  uint32_t hrs = 1;                       // its replica groups per block
   int fused = 0;                          // sweeps without refresh: 0 two-pass, 1/2 one-pass kernel
   uint32_t fty = 0, ftz = 0, fny = 0;     // its tile: rows, planes, thread rows
   int fyhalo = 1;

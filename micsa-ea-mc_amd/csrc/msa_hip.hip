@@ -37,8 +37,9 @@ struct SweepArgs {
   ulonglong2 *dm_out;
   Geo g;
   int layer, colour;
-  uint32_t nr;              // rows (y, z) per tile; a block is nr x L/2 threads
-  uint32_t mLh, mL;         // div_recip reciprocals of L/2 and L
+  uint32_t nr;              // rows (y, z) per tile
+  uint32_t rs;              // replica groups (ONE form: R): a block is rs x nr x L/2 threads
+  uint32_t mLh, mL, mG;     // div_recip reciprocals of L/2, L and nr L/2
   const uint4 *ctr;         // refresh: [R] counters of this refresh
   const uint32_t *key;      // refresh: [R]
   const msa::FineLut *lut;  // refresh: [2] plane-0 and plane-1 tables of this refresh
@@ -63,12 +64,15 @@ __device__ __forceinline__ uint32_t div_recip(uint32_t n, uint32_t m) { return _
 // bx + 4 by + 16 bz of the random words of its MSC word
 // (a + ML be + ML^2 g) / 2 and replica (packing_u ini.c:317-337, ML = L/4),
 // the same bit for every sample, as rand_h is indexed in main.cu:877-879.
-// Thread = one site of the colour, all R replicas (J loaded once).  Blocks
+// Thread = one site of the colour and either all R replicas (J loaded once)
+// or, in the ONE form, the one replica of its replica group: a block of the
+// ONE form is R groups of whole wavefronts, so a small lattice still fills
+// its CUs with waves (ONE with R = 1 is the plain one-replica kernel).  Blocks
 // are persistent: each stages the two search tables of the refresh in LDS
 // once (2 x 34 KiB) and then walks tiles of nr rows.  Used for every sweep
 // with a refresh (VALU-bound: Philox and the table search), for the
 // two-pass mode and for thermal sweeps.
-template <int KIND>
+template <int KIND, bool ONE>
 __global__ void __launch_bounds__(1024) k_halfsweep(SweepArgs a) {
   constexpr bool REFRESH = KIND == kRefresh;
   extern __shared__ uint4 s_tab[];
@@ -82,7 +86,11 @@ __global__ void __launch_bounds__(1024) k_halfsweep(SweepArgs a) {
   }
   const msa::LutView t0 = msa::lut_view(tabs), t1 = msa::lut_view(tabs + 1);
   const uint32_t Lh = g.Lh, L = g.L;
-  const uint32_t rloc = div_recip(threadIdx.x, a.mLh), i = threadIdx.x - rloc * Lh;
+  // replica group: wave-uniform (groups are whole wavefronts), so replica
+  // offsets stay scalar
+  const uint32_t rq = ONE ? __builtin_amdgcn_readfirstlane(div_recip(threadIdx.x, a.mG)) : 0;
+  const uint32_t tl = threadIdx.x - rq * (a.nr * Lh);
+  const uint32_t rloc = div_recip(tl, a.mLh), i = tl - rloc * Lh;
   const uint32_t nrows = L * L, ntiles = (nrows + a.nr - 1) / a.nr;
   const int col = a.colour;
   int dx, dy, dz;
@@ -115,7 +123,7 @@ __global__ void __launch_bounds__(1024) k_halfsweep(SweepArgs a) {
       const uint32_t bit = bx + 4 * by + 16 * bz;
       J0 = Jc[h]; J1 = Jc[H + h]; J2 = Jc[2 * H + h];
       J3 = Jo[hxm]; J4 = Jo[H + hym]; J5 = Jo[2 * H + hzm];
-      for (uint32_t r = 0; r < g.R; ++r) {
+      for (uint32_t r = rq; r < (ONE ? rq + 1 : g.R); ++r) {
         const uint32_t w = (ms >> 1) * g.R + r;
         const uint64_t b0 = 0ull - ((a.rnd[w] >> bit) & 1ull);
         const uint64_t b1 = 0ull - ((a.rnd[a.rT + w] >> bit) & 1ull);
@@ -128,12 +136,12 @@ __global__ void __launch_bounds__(1024) k_halfsweep(SweepArgs a) {
       }
       continue;
     }
-    for (uint32_t r = 0; r < g.R; ++r) {
+    for (uint32_t r = rq; r < (ONE ? rq + 1 : g.R); ++r) {
       const uint64_t *So = a.sp_nb + (uint64_t)(2 * r + (col ^ 1)) * g.H;
       const uint64_t dbase = ((uint64_t)(r * g.nc + a.layer) * 2 + col) * g.H;
       uint64_t sown, n0, n1, n2, n3, n4, n5;
       auto load = [&]() {
-        if (r == 0) {
+        if (r == rq) {
           J0 = Jc[h]; J1 = Jc[H + h]; J2 = Jc[2 * H + h];
           J3 = Jo[hxm]; J4 = Jo[H + hym]; J5 = Jo[2 * H + hzm];
         }
@@ -900,23 +908,52 @@ extern "C" int64_t msa_footprint_bytes(const msa_geom *geom) {
   return (int64_t)state_bytes(geom);
 }
 
-// Tile of the two-pass kernel: nr rows of L/2 site pairs per block.  Blocks
-// are persistent, one per CU (the refresh kernel's registers allow 16 waves
-// per CU, one block of up to 1024 threads), so the block is made as large as
-// possible (most waves per CU to hide the latency of the table searches)
-// while every CU still gets a tile, and never below one wavefront.
+// Tile of the two-pass kernel: nr rows of L/2 site pairs per block, rs
+// replica groups.  Blocks are persistent (as many as the CUs hold at once,
+// by the occupancy of the refresh kernel with its two LDS tables) and walk
+// tiles of nr rows.  The refresh is VALU work with long dependency chains and
+// LDS searches, so the choice maximises the waves resident per CU: over the
+// form with all R replicas per thread and the ONE form with the replicas on
+// separate groups of whole wavefronts (fewer registers, and on small
+// lattices twice the threads: L = 64, R = 2 fills a CU with 1024 threads
+// where one thread per site gives 512), and over the rows per tile; ties go
+// to the larger tile and then to the ONE form.
 static void pick_half(msa_ctx *c) {
-  const uint32_t Lh = c->g.Lh, rows = c->g.L * c->g.L;
+  const uint32_t Lh = c->g.Lh, rows = c->g.L * c->g.L, R = c->g.R;
   int ncu = 0;
   if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || ncu < 1)
     ncu = 256;
-  const uint32_t nr_max = 1024 / Lh > 0 ? 1024 / Lh : 1;
-  const uint32_t nr_min = (64 + Lh - 1) / Lh < nr_max ? (64 + Lh - 1) / Lh : nr_max;
-  uint32_t nr = nr_max;
-  while (nr > nr_min && (rows + nr - 1) / nr < (uint32_t)ncu) --nr;
-  const uint32_t tiles = (rows + nr - 1) / nr;
-  c->hnr = nr;
-  c->hgrid = tiles < (uint32_t)ncu ? tiles : (uint32_t)ncu;
+  double best = -1;
+  c->hnr = 1;
+  c->hrs = R == 1 ? 1 : 0;
+  c->hgrid = 1;
+  for (int form = 0; form < 2; ++form) {
+    const bool one = form == 1 || R == 1;
+    const uint32_t rs = form == 1 ? R : 1;
+    if (form == 1 && R == 1) continue;
+    uint32_t step = 1;                       // ONE form: a group is whole wavefronts
+    if (form == 1)
+      while ((step * Lh) % 64) ++step;
+    for (uint32_t nr = step; (uint64_t)nr * Lh * rs <= 1024; nr += step) {
+      const uint32_t threads = nr * Lh * rs;
+      int bpc = 0;
+      const void *f = one ? reinterpret_cast<const void *>(&k_halfsweep<kRefresh, true>)
+                          : reinterpret_cast<const void *>(&k_halfsweep<kRefresh, false>);
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, f, (int)threads, kRefreshLds) != hipSuccess ||
+          bpc < 1)
+        bpc = 1;
+      const uint64_t tiles = (rows + nr - 1) / nr;
+      const uint64_t blocks = tiles < (uint64_t)ncu * bpc ? tiles : (uint64_t)ncu * bpc;
+      const double waves = (double)blocks * threads / 64.0 / ncu;   // active lanes, in waves
+      if (waves >= best) {
+        best = waves;
+        c->hnr = nr;
+        c->hrs = rs;
+        c->hgrid = (uint32_t)blocks;
+      }
+    }
+  }
+  if (c->hrs == 0) c->hrs = 1;
 }
 
 static bool fused_supported_R(uint32_t R) { return R == 1 || R == 2 || R == 4; }
@@ -1032,7 +1069,9 @@ extern "C" int msa_create(int device, const msa_geom *geom, msa_ctx **out) {
     return cleanup(msa_fail(MSA_ERR_HIP, "initial clear failed"));
   // the refresh kernel stages two search tables (68 KiB) in LDS; the
   // sweep-pair kernel's buffers take up to a CU's whole LDS
-  if (hipFuncSetAttribute(reinterpret_cast<const void *>(&k_halfsweep<kRefresh>),
+  if (hipFuncSetAttribute(reinterpret_cast<const void *>(&k_halfsweep<kRefresh, true>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)kRefreshLds) != hipSuccess ||
+      hipFuncSetAttribute(reinterpret_cast<const void *>(&k_halfsweep<kRefresh, false>),
                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)kRefreshLds) != hipSuccess ||
       hipFuncSetAttribute(reinterpret_cast<const void *>(&k_sweep_pair),
                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPairLdsMax) != hipSuccess)
@@ -1372,23 +1411,30 @@ static int launch_run(msa_ctx *c, uint64_t t0, int64_t niter, int64_t every) {
         a.g = c->g;
         a.layer = (int)l;
         a.nr = c->hnr;
+        a.rs = c->hrs;
         a.mLh = div_recip_of(c->g.Lh);
         a.mL = div_recip_of(c->g.L);
+        a.mG = div_recip_of(c->hnr * c->g.Lh);
         if (k >= 0) {
           a.ctr = c->ctr + (size_t)k * c->g.R;
           a.key = c->key;
           a.lut = c->luts + 2 * (size_t)c->lut_idx[k];
         }
-        const dim3 grid(c->hgrid), block(c->hnr * c->g.Lh);
+        const dim3 grid(c->hgrid), block(c->hnr * c->g.Lh * c->hrs);
         for (int col = 0; col < 2; ++col) {
           a.colour = col;
           a.sp_own = A;
           a.sp_nb = col ? B : A;
           a.sp_out = B;
-          if (k >= 0)
-            hipLaunchKernelGGL(k_halfsweep<kRefresh>, grid, block, kRefreshLds, c->stream, a);
+          const bool one = c->hrs == c->g.R;   // also R = 1
+          if (k >= 0 && one)
+            hipLaunchKernelGGL((k_halfsweep<kRefresh, true>), grid, block, kRefreshLds, c->stream, a);
+          else if (k >= 0)
+            hipLaunchKernelGGL((k_halfsweep<kRefresh, false>), grid, block, kRefreshLds, c->stream, a);
+          else if (one)
+            hipLaunchKernelGGL((k_halfsweep<kMicro, true>), grid, block, 0, c->stream, a);
           else
-            hipLaunchKernelGGL(k_halfsweep<kMicro>, grid, block, 0, c->stream, a);
+            hipLaunchKernelGGL((k_halfsweep<kMicro, false>), grid, block, 0, c->stream, a);
         }
       }
       c->scur ^= 1;
@@ -1448,11 +1494,13 @@ extern "C" int msa_run_thermal(msa_ctx *c, int64_t niter, const msa_lut *lut, co
     a.jp = c->jp;
     a.g = g;
     a.nr = c->hnr;
+    a.rs = c->hrs;
     a.mLh = div_recip_of(g.Lh);
     a.mL = div_recip_of(g.L);
+    a.mG = div_recip_of(c->hnr * g.Lh);
     a.rnd = c->th_rnd;
     a.rT = T;
-    const dim3 grid(c->hgrid), block(c->hnr * g.Lh);
+    const dim3 grid(c->hgrid), block(c->hnr * g.Lh * c->hrs);
     for (int64_t t = 0; t < niter && !rc; ++t) {
       const uint64_t *A = c->sp[c->scur];
       uint64_t *B = c->sp[c->scur ^ 1];
@@ -1464,7 +1512,10 @@ extern "C" int msa_run_thermal(msa_ctx *c, int64_t niter, const msa_lut *lut, co
         a.sp_own = A;
         a.sp_nb = col ? B : A;
         a.sp_out = B;
-        hipLaunchKernelGGL(k_halfsweep<kThermal>, grid, block, 0, c->stream, a);
+        if (c->hrs == g.R)
+          hipLaunchKernelGGL((k_halfsweep<kThermal, true>), grid, block, 0, c->stream, a);
+        else
+          hipLaunchKernelGGL((k_halfsweep<kThermal, false>), grid, block, 0, c->stream, a);
       }
       if (hipGetLastError() != hipSuccess) rc = msa_fail(MSA_ERR_HIP, "thermal launch");
       c->scur ^= 1;

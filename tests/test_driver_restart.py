@@ -188,10 +188,14 @@ def assert_same_outputs(outA, outB, lanes, R, sample=3):
 
 
 # The annealed case once ended with SIGSEGV in the driver process after it had
-# printed its final line (run B, maxtime 32) on an MI355X.  The driver's host
-# code is clean under AddressSanitizer and UBSan with a CPU stand-in of the
-# device layer, for all three cases and their restarts; no cause is found, so
-# the case is not run until one is.
+# printed its final line (run B, maxtime 32) on an MI355X.  The driver, the
+# host library and the device library's own host code (libmsa_hip.so's
+# sources compiled host-only, against a CPU stand-in of the HIP runtime calls)
+# are clean under AddressSanitizer, UBSan and LeakSanitizer for this exact
+# case -- run B, its flag-2 restart and the uninterrupted run, through
+# msa_destroy and process exit -- and for the other two cases; what remains
+# untested is the HIP runtime's own teardown.  No cause is shown, so the case
+# is not run until one is.
 ANNEALED_EXIT = pytest.mark.skip(reason="driver process ended with SIGSEGV after its last line; "
                                         "cause not found")
 
