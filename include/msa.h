@@ -126,6 +126,19 @@ int msa_run_timed(msa_ctx *ctx, uint64_t t0, int64_t niter, int64_t refresh_ever
 int msa_run_thermal(msa_ctx *ctx, int64_t niter, const msa_lut *lut, const msa_u32x4 *ctr,
                     const uint32_t *key);
 
+/* msa_run_thermal without allocating device memory: the same iterations,
+ * bit-identical results, with the search table, counters, keys and thermal
+ * random words in a fixed device array of 16 MiB that the library holds on
+ * every GPU it runs on (set up by msa_create, kept until the process ends).
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Returns MSA_ERR_RANGE when a call needs more than the array
+ * (about 24 V R / 128 + 32 R niter bytes plus 40 KiB).  All calls on one GPU
+ * share the array: calls that overlap on one GPU (other host threads, other
+ * contexts of the same device) race.  Blocks until done. */
+int msa_run_thermal_noalloc(msa_ctx *ctx, int64_t niter, const msa_lut *lut, const msa_u32x4 *ctr,
+                            const uint32_t *key);
+
 /* Integer observables per sample lane (each array [..][64], NULL to skip):
  *   unsat[R][64]   unsatisfied bonds; energy h_Ener = 3V - 2*unsat
  *                  (d_computeEne main.cu:745-815, Measure_Energy_Spins main.cu:315-336)

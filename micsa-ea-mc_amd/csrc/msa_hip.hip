@@ -7,6 +7,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "msa.h"
@@ -220,6 +221,23 @@ __global__ void __launch_bounds__(256) k_gen_rndbits(const uint4 *ctr, const uin
   out[2 * (uint64_t)T + tid] = (w[4] << 32) | w[5];
 }
 
+// Upload `ntab` heat-bath tables to raw (device) and build their search
+// tables in out (device); asynchronous on `st`.
+static int fine_luts_into(hipStream_t st, size_t ntab, const msa_lut *host, msa_lut *raw,
+                          msa::FineLut *out) {
+  if (hipMemcpyAsync(raw, host, sizeof(msa_lut) * ntab, hipMemcpyHostToDevice, st) != hipSuccess)
+    return msa_fail(MSA_ERR_HIP, "LUT upload");
+  // gridDim.y is at most 65535: launches of up to that many tables
+  constexpr uint32_t kMaxY = 65535;
+  for (size_t first = 0; first < ntab; first += kMaxY) {
+    const uint32_t n = (uint32_t)(ntab - first < kMaxY ? ntab - first : kMaxY);
+    hipLaunchKernelGGL(k_fine_luts, dim3((1u << msa::kPreBits) / 256 + 1, n), dim3(256), 0, st,
+                       raw, (uint32_t)first, (uint32_t)ntab, out);
+  }
+  if (hipGetLastError() != hipSuccess) return msa_fail(MSA_ERR_HIP, "search table construction");
+  return MSA_OK;
+}
+
 int msa_build_fine_luts(hipStream_t st, int32_t nlut, const msa_lut *host, msa::FineLut **out) {
   *out = nullptr;
   msa_lut *raw = nullptr;
@@ -230,20 +248,9 @@ int msa_build_fine_luts(hipStream_t st, int32_t nlut, const msa_lut *host, msa::
     (void)hipFree(raw);
     return msa_fail(MSA_ERR_NOMEM, "search tables of %d pairs", nlut);
   }
-  int rc = MSA_OK;
-  if (hipMemcpyAsync(raw, host, sizeof(msa_lut) * ntab, hipMemcpyHostToDevice, st) != hipSuccess)
-    rc = msa_fail(MSA_ERR_HIP, "LUT upload");
-  if (!rc) {
-    // gridDim.y is at most 65535: launches of up to that many tables
-    constexpr uint32_t kMaxY = 65535;
-    for (size_t first = 0; first < ntab; first += kMaxY) {
-      const uint32_t n = (uint32_t)(ntab - first < kMaxY ? ntab - first : kMaxY);
-      hipLaunchKernelGGL(k_fine_luts, dim3((1u << msa::kPreBits) / 256 + 1, n), dim3(256), 0, st,
-                         raw, (uint32_t)first, (uint32_t)ntab, *out);
-    }
-    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
-      rc = msa_fail(MSA_ERR_HIP, "search table construction");
-  }
+  int rc = fine_luts_into(st, ntab, host, raw, *out);
+  if (!rc && hipStreamSynchronize(st) != hipSuccess)
+    rc = msa_fail(MSA_ERR_HIP, "search table construction");
   (void)hipFree(raw);
   if (rc) {
     (void)hipFree(*out);
@@ -1021,6 +1028,8 @@ static void pick_fused(msa_ctx *c) {
   if (best_ty) set_fused(c, narrow_ok(g) ? 2 : 1, best_ty, best_tz);
 }
 
+static int thermal_scratch_init(int device);
+
 extern "C" int msa_create(int device, const msa_geom *geom, msa_ctx **out) {
   if (!out) return msa_fail(MSA_ERR_ARG, "out is NULL");
   *out = nullptr;
@@ -1078,6 +1087,7 @@ extern "C" int msa_create(int device, const msa_geom *geom, msa_ctx **out) {
     return cleanup(msa_fail(MSA_ERR_HIP, "kernel LDS attribute"));
   pick_half(c);
   pick_fused(c);
+  if ((rc = thermal_scratch_init(device))) return cleanup(rc);
   *out = c;
   return MSA_OK;
 }
@@ -1464,13 +1474,61 @@ extern "C" int msa_run_timed(msa_ctx *c, uint64_t t0, int64_t niter, int64_t ref
   return MSA_OK;
 }
 
-extern "C" int msa_run_thermal(msa_ctx *c, int64_t niter, const msa_lut *lut, const msa_u32x4 *ctr,
-                               const uint32_t *key) {
+static int thermal_args_ok(msa_ctx *c, int64_t niter, const msa_lut *lut, const msa_u32x4 *ctr,
+                           const uint32_t *key) {
   if (!c || !lut || (niter > 0 && (!ctr || !key))) return msa_fail(MSA_ERR_ARG, "NULL argument");
   if (niter < 0) return msa_fail(MSA_ERR_ARG, "niter=%lld", (long long)niter);
   if (c->g.L % 8) return msa_fail(MSA_ERR_ARG, "thermal sweeps use the MSC word map: L=%u must be a multiple of 8", c->g.L);
   if (!lut_valid(*lut)) return msa_fail(MSA_ERR_ARG, "heat-bath LUT is malformed");
-  if (niter == 0) return MSA_OK;
+  return MSA_OK;
+}
+
+// Monte_Carlo with flag = 0 (main.cu:202-252) niter times: per iteration and
+// parity p, the random words of parity p (counters dctr[(t*2 + p)*R ..]) and
+// the Metropolis update of colour p, on the context stream; then wait.
+static int thermal_sweeps(msa_ctx *c, int64_t niter, const msa::FineLut *tab, const uint4 *dctr,
+                          const uint32_t *dkey, uint64_t *rnd) {
+  const Geo &g = c->g;
+  const uint32_t T = (uint32_t)(g.V / 128 * g.R);
+  SweepArgs a{};
+  a.jp = c->jp;
+  a.g = g;
+  a.nr = c->hnr;
+  a.rs = c->hrs;
+  a.mLh = div_recip_of(g.Lh);
+  a.mL = div_recip_of(g.L);
+  a.mG = div_recip_of(c->hnr * g.Lh);
+  a.rnd = rnd;
+  a.rT = T;
+  const dim3 grid(c->hgrid), block(c->hnr * g.Lh * c->hrs);
+  int rc = MSA_OK;
+  for (int64_t t = 0; t < niter && !rc; ++t) {
+    const uint64_t *A = c->sp[c->scur];
+    uint64_t *B = c->sp[c->scur ^ 1];
+    for (int col = 0; col < 2; ++col) {
+      // Monte_Carlo flag = 0 (main.cu:202-252): bits of parity col, then colour col
+      hipLaunchKernelGGL(k_gen_rndbits, dim3((T + 255) / 256), dim3(256), 0, c->stream,
+                         dctr + ((size_t)t * 2 + col) * g.R, dkey, tab, g.R, T, rnd);
+      a.colour = col;
+      a.sp_own = A;
+      a.sp_nb = col ? B : A;
+      a.sp_out = B;
+      if (c->hrs == g.R)
+        hipLaunchKernelGGL((k_halfsweep<kThermal, true>), grid, block, 0, c->stream, a);
+      else
+        hipLaunchKernelGGL((k_halfsweep<kThermal, false>), grid, block, 0, c->stream, a);
+    }
+    if (hipGetLastError() != hipSuccess) rc = msa_fail(MSA_ERR_HIP, "thermal launch");
+    c->scur ^= 1;
+  }
+  if (hipStreamSynchronize(c->stream) != hipSuccess && !rc) rc = msa_fail(MSA_ERR_HIP, "thermal sweeps");
+  return rc;
+}
+
+extern "C" int msa_run_thermal(msa_ctx *c, int64_t niter, const msa_lut *lut, const msa_u32x4 *ctr,
+                               const uint32_t *key) {
+  int rc = thermal_args_ok(c, niter, lut, ctr, key);
+  if (rc || niter == 0) return rc;
   HIPCHK(hipSetDevice(c->device));
   const Geo &g = c->g;
   const uint32_t T = (uint32_t)(g.V / 128 * g.R);
@@ -1479,7 +1537,7 @@ extern "C" int msa_run_thermal(msa_ctx *c, int64_t niter, const msa_lut *lut, co
   // the table twice: msa_build_fine_luts builds pairs
   const msa_lut pair[2] = {*lut, *lut};
   msa::FineLut *tab = nullptr;
-  int rc = msa_build_fine_luts(c->stream, 1, pair, &tab);
+  rc = msa_build_fine_luts(c->stream, 1, pair, &tab);
   if (rc) return rc;
   uint4 *dctr = nullptr;
   uint32_t *dkey = nullptr;
@@ -1490,43 +1548,91 @@ extern "C" int msa_run_thermal(msa_ctx *c, int64_t niter, const msa_lut *lut, co
              hipMemcpyAsync(dkey, key, sizeof(uint32_t) * g.R, hipMemcpyHostToDevice, c->stream) != hipSuccess) {
     rc = msa_fail(MSA_ERR_HIP, "thermal counter upload");
   } else {
-    SweepArgs a{};
-    a.jp = c->jp;
-    a.g = g;
-    a.nr = c->hnr;
-    a.rs = c->hrs;
-    a.mLh = div_recip_of(g.Lh);
-    a.mL = div_recip_of(g.L);
-    a.mG = div_recip_of(c->hnr * g.Lh);
-    a.rnd = c->th_rnd;
-    a.rT = T;
-    const dim3 grid(c->hgrid), block(c->hnr * g.Lh * c->hrs);
-    for (int64_t t = 0; t < niter && !rc; ++t) {
-      const uint64_t *A = c->sp[c->scur];
-      uint64_t *B = c->sp[c->scur ^ 1];
-      for (int col = 0; col < 2; ++col) {
-        // Monte_Carlo flag = 0 (main.cu:202-252): bits of parity col, then colour col
-        hipLaunchKernelGGL(k_gen_rndbits, dim3((T + 255) / 256), dim3(256), 0, c->stream,
-                           dctr + ((size_t)t * 2 + col) * g.R, dkey, tab, g.R, T, c->th_rnd);
-        a.colour = col;
-        a.sp_own = A;
-        a.sp_nb = col ? B : A;
-        a.sp_out = B;
-        if (c->hrs == g.R)
-          hipLaunchKernelGGL((k_halfsweep<kThermal, true>), grid, block, 0, c->stream, a);
-        else
-          hipLaunchKernelGGL((k_halfsweep<kThermal, false>), grid, block, 0, c->stream, a);
-      }
-      if (hipGetLastError() != hipSuccess) rc = msa_fail(MSA_ERR_HIP, "thermal launch");
-      c->scur ^= 1;
-    }
-    if (hipStreamSynchronize(c->stream) != hipSuccess && !rc) rc = msa_fail(MSA_ERR_HIP, "thermal sweeps");
+    rc = thermal_sweeps(c, niter, tab, dctr, dkey, c->th_rnd);
   }
   (void)hipStreamSynchronize(c->stream);
   (void)hipFree(dctr);
   (void)hipFree(dkey);
   (void)hipFree(tab);
   return rc;
+}
+
+// ------------------------------------------------- fixed device scratch --
+// The device memory msa_run_thermal allocates on every call (the raw LUT and
+// its search table, the counters and keys, the thermal random words) lives
+// here for msa_run_thermal_noalloc, which calls no allocator.  Every GPU the
+// library runs on holds its own copy until the process ends; host code never
+// uses the symbol itself (on the host it is a stand-in, not a device
+// address) but the address hipGetSymbolAddress gives for the current device,
+// fetched once per device by msa_create (thermal_scratch_init).
+constexpr size_t kScratchBytes = 16u << 20;
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_dev[kMaxDevices];
+
+// With `device` current: the address of this device's scratch, once.
+static int thermal_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices)
+    return msa_fail(MSA_ERR_ARG, "device %d: the fixed scratch covers devices 0..%d", device, kMaxDevices - 1);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_dev[device]) return MSA_OK;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return msa_fail(MSA_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", device,
+                    hipGetErrorString(e));
+  g_scratch_dev[device] = p;
+  return MSA_OK;
+}
+
+static size_t up256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+// Scratch layout of one call: raw LUT | search table | keys | counters | random words.
+static size_t thermal_scratch_bytes(const Geo &g, int64_t niter) {
+  const size_t T = g.V / 128 * g.R;
+  return up256(sizeof(msa_lut)) + up256(sizeof(msa::FineLut)) + up256(4 * (size_t)g.R) +
+         up256(16 * (size_t)niter * 2 * g.R) + up256(24 * T);
+}
+
+extern "C" int msa_run_thermal_noalloc(msa_ctx *c, int64_t niter, const msa_lut *lut,
+                                       const msa_u32x4 *ctr, const uint32_t *key) {
+  int rc = thermal_args_ok(c, niter, lut, ctr, key);
+  if (rc || niter == 0) return rc;
+  const size_t need = thermal_scratch_bytes(c->g, niter);
+  if (need > kScratchBytes)
+    return msa_fail(MSA_ERR_RANGE,
+                    "msa_run_thermal_noalloc: L=%u R=%u niter=%lld needs %zu bytes of the fixed "
+                    "scratch, which holds %zu", c->g.L, c->g.R, (long long)niter, need, kScratchBytes);
+  HIPCHK(hipSetDevice(c->device));
+  uint8_t *base;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    base = static_cast<uint8_t *>(g_scratch_dev[c->device]);
+  }
+  if (!base) return msa_fail(MSA_ERR_STATE, "the fixed scratch of device %d is not set up", c->device);
+  const Geo &g = c->g;
+  const size_t T = g.V / 128 * g.R;
+  msa_lut *raw = reinterpret_cast<msa_lut *>(base);
+  uint8_t *q = base + up256(sizeof(msa_lut));
+  msa::FineLut *tab = reinterpret_cast<msa::FineLut *>(q);
+  q += up256(sizeof(msa::FineLut));
+  uint32_t *dkey = reinterpret_cast<uint32_t *>(q);
+  q += up256(4 * (size_t)g.R);
+  uint4 *dctr = reinterpret_cast<uint4 *>(q);
+  q += up256(16 * (size_t)niter * 2 * g.R);
+  uint64_t *rnd = reinterpret_cast<uint64_t *>(q);
+  (void)T;
+  // every scratch byte a kernel reads is written first in stream order
+  // (LUT, table, keys, counters by the copies and k_fine_luts; the random
+  // words by k_gen_rndbits), so nothing needs clearing
+  HIPCHK(hipStreamSynchronize(c->stream));
+  rc = fine_luts_into(c->stream, 1, lut, raw, tab);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(dkey, key, sizeof(uint32_t) * g.R, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(dctr, ctr, sizeof(uint4) * (size_t)niter * 2 * g.R, hipMemcpyHostToDevice,
+                        c->stream));
+  return thermal_sweeps(c, niter, tab, dctr, dkey, rnd);
 }
 
 extern "C" int msa_measure(msa_ctx *c, int32_t *unsat, int32_t *demon, int32_t *q,

@@ -71,6 +71,7 @@ def hip():
         L.msa_set_tiling.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32]
         L.msa_get_tiling.argtypes = [vp, i32p, i32p, i32p]
         L.msa_run_thermal.argtypes = [vp, C.c_int64, C.c_void_p, u32p, u32p]
+        L.msa_run_thermal_noalloc.argtypes = [vp, C.c_int64, C.c_void_p, u32p, u32p]
         L.msa_set_walker_streams.argtypes = [vp, C.c_int32, i32p, C.c_void_p, C.c_int32, u64p]
         L.msa_get_walker_streams.argtypes = [vp, C.c_void_p]
         bp = C.c_void_p
@@ -199,6 +200,17 @@ class Lattice:
         key = np.ascontiguousarray(key, np.uint32)
         _check(self.lib.msa_run_thermal(self.ctx, ctr.shape[0], raw.ctypes.data_as(C.c_void_p),
                                         _p(ctr, u32p), _p(key, u32p)))
+
+    def run_thermal_noalloc(self, lut, ctr, key):
+        """msa_run_thermal_noalloc: the thermalization without device
+        allocation (scratch in the library's fixed device array)."""
+        raw = np.ascontiguousarray(np.frombuffer(bytes(lut), np.uint8))
+        assert raw.size == LUT_BYTES
+        ctr = np.ascontiguousarray(ctr, np.uint32).reshape(-1, 2, self.R, 4)
+        key = np.ascontiguousarray(key, np.uint32)
+        _check(self.lib.msa_run_thermal_noalloc(self.ctx, ctr.shape[0],
+                                                raw.ctypes.data_as(C.c_void_p),
+                                                _p(ctr, u32p), _p(key, u32p)))
 
     def run_timed(self, t0, niter, refresh_every):
         ms = C.c_float()
