@@ -166,6 +166,20 @@ def progress(msg):
     print("bench.py: " + msg, file=sys.stderr, flush=True)
 
 
+class Ticker:
+    """progress() at most every `every` seconds, for long loops."""
+
+    def __init__(self, every=30.0):
+        self.every = every
+        self.last = time.perf_counter()
+
+    def __call__(self, msg):
+        now = time.perf_counter()
+        if now - self.last >= self.every:
+            self.last = now
+            progress(msg)
+
+
 def splitmix64(x):
     x = (x + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
     z = x
@@ -200,7 +214,9 @@ def plain_seconds(pkg, tables, reqs, jobs):
     after the other (its setup not counted), same tables and seeds as the
     batched and served forms."""
     total = 0.0
-    for req, job in zip(reqs, jobs):
+    tick = Ticker()
+    for n, (req, job) in enumerate(zip(reqs, jobs)):
+        tick("plain form: %d of %d lattices" % (n, len(reqs)))
         lat = pkg.Lattice(req.L, req.R, 1)
         lat.init_philox(req.seed)
         lat.set_luts(tables.luts)
@@ -248,9 +264,16 @@ def batched_vs_plain(pkg, n_lat, len_div, block=None, chunk=64):
     b.sync()
     b.run(0)
     b.sync()
+    # in segments of 8192 iterations, a progress line after each (the
+    # segment boundaries cost one sync each)
     t0 = time.perf_counter()
-    b.run(max(n for _, n in lats))
-    b.sync()
+    left, tick = max(n for _, n in lats), Ticker()
+    while left > 0:
+        seg = min(left, 8192)
+        b.run(seg)
+        b.sync()
+        left -= seg
+        tick("batched form: %d iterations left" % left)
     t_batch = time.perf_counter() - t0
     t, te = b.progress(len(lats))
     assert (t == te).all()
@@ -284,10 +307,10 @@ def served(pkg, n_req, load, log2_len, max_bytes, epoch=64):
     _, summ = serve.serve(reqs, max_bytes=max_bytes, epoch=epoch, tables=tables, progress=progress)
     # plain: FIFO, one request alone at a time
     tp = []
+    tick = Ticker()
     for r in reqs:
         tp.append(plain_seconds(pkg, tables, [r], [tables.job(r, r.rid)]))
-        if len(tp) % 1000 == 0:
-            progress("plain form: %d of %d requests" % (len(tp), len(reqs)))
+        tick("plain form: %d of %d requests" % (len(tp), len(reqs)))
     finish, lat = 0.0, []
     for r, t in sorted(zip(reqs, tp), key=lambda x: x[0].arrival):
         finish = max(finish, r.arrival) + t
@@ -327,6 +350,8 @@ def main():
     ap.add_argument("--batched-len-div", type=int, default=16,
                     help="anneal lengths {4096,16384,65536} divided by this")
     ap.add_argument("--served", type=int, default=1000, help="requests in the served test (0: skip)")
+    ap.add_argument("--served-len", type=str, default="6,11",
+                    help="log2 of the served requests' lengths, LO,HI (the served configuration: 12,17)")
     ap.add_argument("--served-load", type=float, default=0.8,
                     help="arrival rate as a fraction of the measured batched capacity")
     ap.add_argument("--no-autotune", action="store_true",
@@ -532,8 +557,9 @@ def main():
         out["batched"] = batched_vs_plain(pkg, args.batched, args.batched_len_div)
     if args.served > 0:
         progress("served form: %d requests" % args.served)
-        # lengths 2^6..2^11: the served configuration's 2^12..2^17 sweeps / 64
-        out["served"] = served(pkg, args.served, args.served_load, (6, 11), 8 << 30)
+        # default lengths 2^6..2^11: the served configuration's 2^12..2^17 sweeps / 64
+        lo, hi = (int(x) for x in args.served_len.split(","))
+        out["served"] = served(pkg, args.served, args.served_load, (lo, hi), 8 << 30)
     if not args.no_cpu:
         progress("CPU baseline: %.0f s" % args.cpu_budget)
         out["cpu_baseline"] = cpu_baseline(L, R, nc, args.cpu_budget, args.cpu_threads, every, me)

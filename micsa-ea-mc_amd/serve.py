@@ -137,7 +137,7 @@ def serve(requests, max_bytes, epoch=64, block=None, tables=None, keep_state=Fal
     interpreter lock) while the device runs; a request is admitted, in
     arrival order, once its tables are ready and its footprint fits.
     progress, if given, is called with a message after every 1000th
-    finished request."""
+    finished request and at least every 30 s while requests finish."""
     tables = tables or Tables()
     if block is None:
         block, _ = tune_block(tables, size_sample(sorted({r.L for r in requests})))
@@ -156,6 +156,7 @@ def serve(requests, max_bytes, epoch=64, block=None, tables=None, keep_state=Fal
     max_active = 0
     max_bytes_used = 0
     finishing = []              # slots that reach their length in the run enqueued last
+    last_note = [clock()]       # when progress was last called
 
     def retire(slot):
         nonlocal updates
@@ -169,7 +170,8 @@ def serve(requests, max_bytes, epoch=64, block=None, tables=None, keep_state=Fal
         done[req.rid] = rec
         updates += 64 * req.R * req.L ** 3 * req.niter
         batch.remove(slot)
-        if progress and len(done) % 1000 == 0:
+        if progress and (len(done) % 1000 == 0 or clock() - last_note[0] >= 30.0):
+            last_note[0] = clock()
             progress("%d of %d requests finished" % (len(done), len(reqs)))
 
     try:
