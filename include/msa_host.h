@@ -167,6 +167,12 @@ int  msa_read_samples(const char *path, int n, int *samples, char *err, int errl
 /* annealing schedule file: T0 T1 nsweeps refresh_every (one per line);
  * T0 = T1 = 0 keeps the inverse temperature of beta.dat */
 int  msa_read_schedule(const char *path, msa_sched *s, char *err, int errlen);
+/* changentimes.in (stop, io.c:930-968): "maxtime [t_1 t_2 ...]", the new end
+ * of the run and, when given, the new strictly increasing list of
+ * measurement times.  0: no file; 1: read (*times malloc'd, NULL when only
+ * maxtime is given); 2: no number in the file (removed); -1: bad file. */
+int  msa_read_changentimes(const char *path, unsigned long long *maxtime,
+                           unsigned long long **times, int *ntimes, char *err, int errlen);
 
 /* Mean and error over replicas as Measures does (med.c:13-51):
  * mean of x_r, sqrt((<x^2> - <x>^2) / (n - 1)). */

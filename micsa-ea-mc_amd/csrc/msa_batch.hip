@@ -65,8 +65,10 @@ __device__ __forceinline__ uint32_t div_recip(uint32_t n, uint32_t m) { return _
 // lattice and the lattice refreshes this iteration.  Index arithmetic is
 // 32-bit (every batched lattice has L^3 < 2^31, checked on admission) and
 // divisions by L/2 and L use per-lattice reciprocals.  Held to four waves
-// per SIMD (128 VGPRs), the occupancy the tables' LDS allows anyway.
-template <int BS>
+// per SIMD (128 VGPRs), the occupancy the tables' LDS allows anyway.  R1:
+// every lattice of the launch has one replica (the host checks), so the
+// replica loop is a single pass and needs fewer registers.
+template <int BS, bool R1>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4)))
 k_batch_halfsweep(const BLat *__restrict__ lats, BProg *prog, const BSeg *__restrict__ segs,
                   const uint32_t *__restrict__ seg_start, int colour) {
@@ -134,7 +136,7 @@ k_batch_halfsweep(const BLat *__restrict__ lats, BProg *prog, const BSeg *__rest
       const uint32_t hw = (uint32_t)msa::walk_dest(g, col, 1, 0, 0, i, y, z);   // layer 0 walks along +x
       const uint32_t tid = x + L * row;
       uint64_t J0, J1, J2, J3, J4, J5;
-      for (uint32_t r = 0; r < g.R; ++r) {
+      for (uint32_t r = 0; r < (R1 ? 1u : g.R); ++r) {
         const uint64_t *So = Sn + (uint64_t)(2 * r + (col ^ 1)) * g.H;
         const ulonglong2 *Di = dmi + ((uint64_t)r * 2 + col) * g.H;
         ulonglong2 *Do = dmo + ((uint64_t)r * 2 + col) * g.H;
@@ -227,6 +229,7 @@ struct msa_batch {
   msa::FineLut *luts = nullptr;         // shared search tables, 2 per pair
   int32_t nlut = 0;
   int block = 256;
+  bool r1 = false;                      // launches use the one-replica kernel
   int ncu = 256;                        // compute units of the device
   int32_t chunk = 16;                   // iterations per captured graph
   hipGraph_t graph = nullptr;
@@ -459,41 +462,59 @@ extern "C" int msa_batch_upload_state(msa_batch *b, int32_t slot, const uint64_t
   return msa_upload_state(b->slot[slot], J, spin, d0, d1);
 }
 
-static void batch_launch(msa_batch *b, int col) {
-  const dim3 grid(b->nblocks);
+// The kernel of a block size and replica form.
+static const void *batch_kernel(int block, bool r1) {
+  switch (block) {
+    case 128: return r1 ? reinterpret_cast<const void *>(&k_batch_halfsweep<128, true>)
+                        : reinterpret_cast<const void *>(&k_batch_halfsweep<128, false>);
+    case 512: return r1 ? reinterpret_cast<const void *>(&k_batch_halfsweep<512, true>)
+                        : reinterpret_cast<const void *>(&k_batch_halfsweep<512, false>);
+    case 1024: return r1 ? reinterpret_cast<const void *>(&k_batch_halfsweep<1024, true>)
+                         : reinterpret_cast<const void *>(&k_batch_halfsweep<1024, false>);
+    default: return r1 ? reinterpret_cast<const void *>(&k_batch_halfsweep<256, true>)
+                       : reinterpret_cast<const void *>(&k_batch_halfsweep<256, false>);
+  }
+}
+
+template <int BS>
+static void batch_launch_bs(msa_batch *b, int col) {
+  const dim3 grid(b->nblocks), block(BS);
   const size_t lds = kRefreshLds;
   const BLat *L = b->d_lats;
   BProg *P = b->d_prog;
   const BSeg *sg = b->d_segs;
   const uint32_t *st = b->d_seg_start;
+  if (b->r1)
+    hipLaunchKernelGGL((k_batch_halfsweep<BS, true>), grid, block, lds, b->stream, L, P, sg, st, col);
+  else
+    hipLaunchKernelGGL((k_batch_halfsweep<BS, false>), grid, block, lds, b->stream, L, P, sg, st, col);
+}
+
+static void batch_launch(msa_batch *b, int col) {
   switch (b->block) {
-    case 128: hipLaunchKernelGGL(k_batch_halfsweep<128>, grid, dim3(128), lds, b->stream, L, P, sg, st, col); break;
-    case 512: hipLaunchKernelGGL(k_batch_halfsweep<512>, grid, dim3(512), lds, b->stream, L, P, sg, st, col); break;
-    case 1024: hipLaunchKernelGGL(k_batch_halfsweep<1024>, grid, dim3(1024), lds, b->stream, L, P, sg, st, col); break;
-    default: hipLaunchKernelGGL(k_batch_halfsweep<256>, grid, dim3(256), lds, b->stream, L, P, sg, st, col); break;
+    case 128: batch_launch_bs<128>(b, col); break;
+    case 512: batch_launch_bs<512>(b, col); break;
+    case 1024: batch_launch_bs<1024>(b, col); break;
+    default: batch_launch_bs<256>(b, col); break;
   }
 }
 
 // The batch kernels stage two search tables (68 KiB) in dynamic LDS.
 static int batch_lds_attr() {
-  const void *fns[] = {reinterpret_cast<const void *>(&k_batch_halfsweep<128>),
-                       reinterpret_cast<const void *>(&k_batch_halfsweep<256>),
-                       reinterpret_cast<const void *>(&k_batch_halfsweep<512>),
-                       reinterpret_cast<const void *>(&k_batch_halfsweep<1024>)};
-  for (const void *f : fns)
-    HIPCHK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kRefreshLds));
+  for (int block : {128, 256, 512, 1024})
+    for (bool r1 : {false, true})
+      HIPCHK(hipFuncSetAttribute(batch_kernel(block, r1), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)kRefreshLds));
   return MSA_OK;
 }
 
 // Blocks of the batch kernel with `block` threads that one CU holds at once
 // (registers and the tables' LDS), at least 1.
-static int batch_resident(int block) {
-  const void *f = reinterpret_cast<const void *>(&k_batch_halfsweep<256>);
-  if (block == 128) f = reinterpret_cast<const void *>(&k_batch_halfsweep<128>);
-  if (block == 512) f = reinterpret_cast<const void *>(&k_batch_halfsweep<512>);
-  if (block == 1024) f = reinterpret_cast<const void *>(&k_batch_halfsweep<1024>);
+static int batch_resident(int block, bool r1) {
   int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, block, kRefreshLds) != hipSuccess || n < 1) n = 1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch_kernel(block, r1), block, kRefreshLds) !=
+          hipSuccess || n < 1)
+    n = 1;
   return n;
 }
 
@@ -580,7 +601,12 @@ static int batch_prepare(msa_batch *b) {
       total += L.ntiles;
     }
   }
-  const uint32_t nblocks = (uint32_t)((uint64_t)b->ncu * (uint64_t)batch_resident(b->block));
+  // the one-replica kernel when every lattice of the split has R = 1
+  bool r1 = true;
+  for (uint32_t s : lat) r1 = r1 && b->h_lats[s].g.R == 1;
+  if (r1 != b->r1) b->graph_stale = true;
+  b->r1 = r1;
+  const uint32_t nblocks = (uint32_t)((uint64_t)b->ncu * (uint64_t)batch_resident(b->block, r1));
   if (nblocks != b->nblocks) b->graph_stale = true;
   std::vector<BSeg> segs;
   std::vector<uint32_t> start;

@@ -511,6 +511,77 @@ static void write_checkpoint(const run_files *rf, unsigned long long now,
   free(jan);
 }
 
+/* The demon refresh of iterations 0 .. niter-1 on the device: the
+ * reference's walker streams (refmode; st = their states now) with
+ * get_thresholds at beta.dat, or the Philox stream's counters and LUT pairs
+ * (from the generators at the run's start, refreshes before k_start unused). */
+static void set_refresh(msa_ctx *ctx, const msa_sched *sch, int refmode, int64_t niter, int R,
+                        double beta, int nbits, const int *samples, const msa_walker_stream *st,
+                        int64_t k_start, const msa_prng *base_pr, const msa_xo256 *base_xo,
+                        uint32_t sample, const uint32_t *keys)
+{
+  const int64_t nref = msa_sched_count(sch, niter > 0 ? niter : 1);
+  if (refmode) {
+    /* refresh_walkers (main.cu:340-377): refresh k takes the next segment of
+     * each walker stream, with get_thresholds at beta.dat (ini.c:680-710) */
+    uint64_t *thr = (uint64_t *)xmalloc(sizeof(uint64_t) * 3 * (size_t)nref, "thresholds");
+    msa_thresholds(beta, thr);
+    for (int64_t k = 1; k < nref; k++) memcpy(thr + 3 * k, thr, 3 * sizeof(uint64_t));
+    CHK(msa_set_walker_streams(ctx, nbits, samples, st, (int32_t)nref, thr));
+    free(thr);
+    return;
+  }
+  msa_prng pr[MSA_MAXR];
+  msa_xo256 xo[MSA_MAXR];
+  memcpy(pr, base_pr, sizeof(msa_prng) * R);
+  memcpy(xo, base_xo, sizeof(msa_xo256) * R);
+  msa_u32x4 *ctr = (msa_u32x4 *)xmalloc(sizeof(msa_u32x4) * (size_t)(nref * R), "refresh counters");
+  msa_lut *luts = (msa_lut *)xmalloc(sizeof(msa_lut) * 2 * (size_t)nref, "refresh tables");
+  int32_t *lut_idx = (int32_t *)xmalloc(sizeof(int32_t) * (size_t)nref, "refresh indices");
+  /* one LUT pair per distinct beta: a fixed-beta run builds one */
+  const int npair = msa_sched_tables_resume(sch, k_start, nref, R, pr, xo, sample, ctr, luts, lut_idx);
+  if (npair < 1) die("Problems building the refresh tables (nref=%lld)", (long long)nref);
+  CHK(msa_set_luts(ctx, npair, luts));
+  CHK(msa_set_refresh(ctx, (int32_t)nref, ctr, keys, lut_idx));
+  free(luts);
+  free(ctr);
+  free(lut_idx);
+}
+
+/* stop (io.c:930-968): <dir>changentimes.in, when present, sets a new end of
+ * the run and, when it lists them, new measurement times.  The reference's
+ * reader has no defined behaviour (an uninitialised pointer); here the file
+ * is "maxtime [t_1 t_2 ...]" (msa_read_changentimes), read before the run
+ * and after every measurement, and kept (so it is read again each time); a
+ * file without a number is removed.  Returns 1 when it changed anything. */
+static int change_times(msa_input *in, unsigned long long **times)
+{
+  char name[PATHLEN + 32], e[512];
+  snprintf(name, sizeof name, "%schangentimes.in", g_dir);
+  unsigned long long maxtime = in->maxtime, *nt = NULL;
+  int n = 0;
+  const int rc = msa_read_changentimes(name, &maxtime, &nt, &n, e, sizeof e);
+  if (rc < 0) die("%s", e);
+  if (rc == 2) {
+    printf(" File '%s' exists but holds no number: removed\n", name);
+    fflush(stdout);
+  }
+  if (rc != 1) return 0;
+  int changed = maxtime != in->maxtime;
+  in->maxtime = maxtime;
+  if (n > 0) {
+    changed = changed || n != in->ntimes || memcmp(nt, *times, sizeof *nt * (size_t)n);
+    free(*times);
+    *times = nt;
+    in->ntimes = n;
+  }
+  if (changed) {
+    printf("%s: maxtime %llu, %d measurement times\n", name, in->maxtime, in->ntimes);
+    fflush(stdout);
+  }
+  return changed;
+}
+
 int main(int argc, char **argv)
 {
   if (argc > 1 && (!strcmp(argv[1], "-h") || !strcmp(argv[1], "--help"))) print_help(argv[0]);
@@ -663,6 +734,9 @@ int main(int argc, char **argv)
     rf.walker_p = wpr[0].p;
   }
 
+  /* the reference reads changentimes.in while restoring confs (io.c:149)
+   * and once more before the run (main.cu:107) */
+  change_times(&in, &times);
   unsigned long long start = in.now;
   if (flag == 0) {
     start = read_checkpoint(&rf, J, S, D0, D1, pr, xo, wpr, wxo);
@@ -699,34 +773,14 @@ int main(int argc, char **argv)
   memcpy(base_pr, pr, sizeof pr);
   memcpy(base_xo, xo, sizeof xo);
 
-  const int64_t niter = (int64_t)in.maxtime;
-  const int64_t nref = msa_sched_count(&sch, niter > 0 ? niter : 1);
+  msa_walker_stream *wst = NULL;
   if (refmode) {
-    /* refresh_walkers (main.cu:340-377): refresh k takes the next segment of
-     * each walker stream, with get_thresholds at beta.dat (ini.c:680-710) */
-    msa_walker_stream *st = (msa_walker_stream *)xmalloc(sizeof(msa_walker_stream) * nbits * R,
-                                                          "walker streams");
-    uint64_t *thr = (uint64_t *)xmalloc(sizeof(uint64_t) * 3 * (size_t)nref, "thresholds");
-    for (int k = 0; k < nbits * R; k++) msa_walker_stream_of(&wpr[k], &wxo[k], &st[k]);
-    msa_thresholds(beta, thr);
-    for (int64_t k = 1; k < nref; k++) memcpy(thr + 3 * k, thr, 3 * sizeof(uint64_t));
-    CHK(msa_set_walker_streams(ctx, nbits, samples, st, (int32_t)nref, thr));
-    free(st);
-    free(thr);
-  } else {
-    msa_u32x4 *ctr = (msa_u32x4 *)xmalloc(sizeof(msa_u32x4) * (size_t)(nref * R), "refresh counters");
-    msa_lut *luts = (msa_lut *)xmalloc(sizeof(msa_lut) * 2 * (size_t)nref, "refresh tables");
-    int32_t *lut_idx = (int32_t *)xmalloc(sizeof(int32_t) * (size_t)nref, "refresh indices");
-    /* one LUT pair per distinct beta: a fixed-beta run builds one */
-    const int npair = msa_sched_tables_resume(&sch, k_start, nref, R, pr, xo, (uint32_t)sample,
-                                              ctr, luts, lut_idx);
-    if (npair < 1) die("Problems building the refresh tables (nref=%lld)", (long long)nref);
-    CHK(msa_set_luts(ctx, npair, luts));
-    CHK(msa_set_refresh(ctx, (int32_t)nref, ctr, keys, lut_idx));
-    free(luts);
-    free(ctr);
-    free(lut_idx);
+    wst = (msa_walker_stream *)xmalloc(sizeof(msa_walker_stream) * nbits * R, "walker streams");
+    for (int k = 0; k < nbits * R; k++) msa_walker_stream_of(&wpr[k], &wxo[k], &wst[k]);
   }
+  set_refresh(ctx, &sch, refmode, (int64_t)in.maxtime, R, beta, nbits, samples, wst, k_start,
+              base_pr, base_xo, (uint32_t)sample, keys);
+  unsigned long long tables_to = in.maxtime;   /* iterations the refresh tables cover */
 
   int32_t *unsat = (int32_t *)calloc((size_t)R * 64, sizeof(int32_t));
   int32_t *demon = (int32_t *)calloc((size_t)R * 64, sizeof(int32_t));
@@ -840,8 +894,21 @@ int main(int argc, char **argv)
                        energy, gpr, gxo, wpr, wxo);
     }
     check_signal();
-    if (check_time(max_time, dt0, it + 1 < in.ntimes ? times[it + 1] - iter : 0) ||
-        ++it == in.ntimes)
+    if (change_times(&in, &times)) {
+      /* a new list: continue with its first time after this one; a later
+       * end needs refresh tables that reach it */
+      it = 0;
+      while (it < in.ntimes && times[it] <= iter) it++;
+      if (in.maxtime > tables_to) {
+        if (refmode) CHK(msa_get_walker_streams(ctx, wst));
+        set_refresh(ctx, &sch, refmode, (int64_t)in.maxtime, R, beta, nbits, samples, wst, k_start,
+                    base_pr, base_xo, (uint32_t)sample, keys);
+        tables_to = in.maxtime;
+      }
+    } else {
+      ++it;
+    }
+    if (check_time(max_time, dt0, it < in.ntimes ? times[it] - iter : 0) || it >= in.ntimes)
       break;
   }
   CHK(msa_sync(ctx));
@@ -861,5 +928,6 @@ int main(int argc, char **argv)
   free(D1);
   free(wpr);
   free(wxo);
+  free(wst);
   return 0;
 }

@@ -118,6 +118,61 @@ int msa_read_times(const char *path, int n, unsigned long long *t, char *e, int 
   return 0;
 }
 
+/* The run-time change of a run's end and measurement list (stop, io.c:930-968).
+ * The reference's reader uses an uninitialised pointer for the first value
+ * and then reads that value's count of times, so it has no defined
+ * behaviour to keep; this one reads the first value as the new maxtime and
+ * every further value as the new list of measurement times, which must be
+ * strictly increasing.  Returns 0 when the file does not exist, 1 when read
+ * (*times = a malloc'd list of *ntimes values, or NULL with *ntimes = 0 when
+ * the file holds the maxtime alone), 2 when it holds no number (it is
+ * removed, as the reference removes it), -1 on a bad value or an unsorted
+ * list. */
+int msa_read_changentimes(const char *path, unsigned long long *maxtime, unsigned long long **times,
+                          int *ntimes, char *e, int elen)
+{
+  *times = NULL;
+  *ntimes = 0;
+  FILE *f = fopen(path, "r");
+  if (!f) return 0;
+  unsigned long long v;
+  if (fscanf(f, "%llu", &v) != 1) {
+    fclose(f);
+    remove(path);
+    return 2;
+  }
+  *maxtime = v;
+  int cap = 0, n = 0;
+  unsigned long long *t = NULL;
+  while (fscanf(f, "%llu", &v) == 1) {
+    if (n == cap) {
+      cap = cap ? 2 * cap : 64;
+      unsigned long long *nt = (unsigned long long *)realloc(t, sizeof *t * (size_t)cap);
+      if (!nt) {
+        free(t);
+        fclose(f);
+        return err(e, elen, "Problems allocating memory for list_times");
+      }
+      t = nt;
+    }
+    t[n++] = v;
+  }
+  const int bad = !feof(f);
+  fclose(f);
+  if (bad) {
+    free(t);
+    return err(e, elen, "File %s: value %d is not a number", path, n + 2);
+  }
+  for (int i = 0; i + 1 < n; i++)
+    if (t[i] >= t[i + 1]) {
+      free(t);
+      return err(e, elen, "List of times %s is not sorted.\nList of times must be sorted!", path);
+    }
+  *times = t;
+  *ntimes = n;
+  return 1;
+}
+
 /* read_list_samples (io.c:593-624): empty path = 0..n-1, else n increasing ids */
 int msa_read_samples(const char *path, int n, int *s, char *e, int elen)
 {

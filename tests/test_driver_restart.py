@@ -170,7 +170,7 @@ EVOL = ("evol_E_spins.txt", "evol_E_walkers.txt", "evol_T_walkers.txt", "evol_E_
         "evol_overlap.txt")
 
 
-def assert_same_outputs(outA, outB, lanes, R, sample=3):
+def assert_same_outputs(outA, outB, lanes, R, sample=3, restarted=True):
     for b in lanes:
         ba = os.path.join(sample_dir(outA, sample), "BIT%03d" % b)
         bb = os.path.join(sample_dir(outB, sample), "BIT%03d" % b)
@@ -184,7 +184,7 @@ def assert_same_outputs(outA, outB, lanes, R, sample=3):
             cb = open(os.path.join(bb, "R%02d" % r, "conf"), "rb").read()
             # s_data.flag (bytes 36..39) is 1 in a new run and 0 after a restart
             assert ca[:36] == cb[:36] and ca[40:] == cb[40:], (b, r)
-            assert np.frombuffer(cb[36:40], np.int32)[0] == 0
+            assert np.frombuffer(cb[36:40], np.int32)[0] == (0 if restarted else 1)
 
 
 # The annealed case once ended with SIGSEGV in the driver process after it had
