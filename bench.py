@@ -339,6 +339,9 @@ def main():
     ap.add_argument("--R", type=int, default=2)
     ap.add_argument("--nc", type=int, default=1)
     ap.add_argument("--refresh-every", type=int, default=1024)
+    ap.add_argument("--anneal", type=str, default="1.8,0.6,262144",
+                    help="T0,T1,N: linear anneal T0 -> T1 over N sweeps (default: the L=160 "
+                         "configuration's; the L=64 one is 2.0,0.7,65536)")
     ap.add_argument("--measure-every", type=int, default=1024,
                     help="measure energies and overlaps every this many sweeps inside the timed region (0: never)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
@@ -372,7 +375,8 @@ def main():
     pkg = load_package()
     L, R, nc, every = args.L, args.R, args.nc, args.refresh_every
     V = L ** 3
-    nsched = 1 << 18
+    aT0, aT1, nsched = args.anneal.split(",")
+    aT0, aT1, nsched = float(aT0), float(aT1), int(nsched)
     lat = pkg.Lattice(L, R, nc, device=0)
     lat.init_philox(0x5EED)
     tuned = None
@@ -382,7 +386,7 @@ def main():
         tuned = lat.autotune(16)
     tiling = lat.tiling()
     total = args.warmup + args.steps
-    sched = pkg.schedule(1.8, 0.6, nsched, every)
+    sched = pkg.schedule(aT0, aT1, nsched, every)
     # tables reach two refreshes past the timed window: the refresh block
     # timed after it (below) starts at the first refresh after the window
     ntab = total + 2 * every if every > 0 else total
@@ -503,8 +507,8 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "plain micSA, L=%d periodic 3D EA +-J, 64 samples per uint64 lane, "
-                        "R=%d replicas, nc=%d, anneal T 1.8->0.6 over 2^18 sweeps, demon "
-                        "refresh every %d sweeps" % (L, R, nc, every),
+                        "R=%d replicas, nc=%d, anneal T %g->%g over %d sweeps, demon "
+                        "refresh every %d sweeps" % (L, R, nc, aT0, aT1, nsched, every),
             "L": L, "R": R, "nc": nc, "lanes": 64, "refresh_every": every,
             "sweeps_timed": args.steps, "refreshes_timed": nref,
             "measure_every": me, "measurements_timed": nmeas,
