@@ -8,16 +8,21 @@ finished ones are retired, freeing their memory for the queue.  All state
 (lattice, demons, counters, tables) stays on the device from admission to
 retirement.  Every result is bit-identical to the same request run alone,
 because each lattice's update depends only on its own state and tables.
+The loop itself is native (driver/msa_serve.c); this module writes its
+request file, reads its results, and holds the plain-path reference run and
+the batch block-size autotuner.
 
 No reference counterpart: the reference runs one sample set per process.
 """
-import concurrent.futures
 import dataclasses
+import os
 import time
 
 import numpy as np
 
 from . import msa
+
+SERVE_BIN = os.path.join(msa.PKG, "bin", "msa_serve")
 
 TGRID = 1.0 / 1024          # temperature grid of the shared LUT table
 T_LO, T_HI = 0.5, 2.5
@@ -128,102 +133,60 @@ def run_alone(req, tables, jobid):
         lat.close()
 
 
-def serve(requests, max_bytes, epoch=64, block=None, tables=None, keep_state=False, clock=None,
-          workers=4, progress=None):
-    """Run the stream; returns (per-request dicts, summary dict).  block=None
-    picks the batch kernel's block size with tune_block on one request per
-    lattice size of the stream.  The refresh tables of arrived requests are
-    built by `workers` host threads (the host library releases the
-    interpreter lock) while the device runs; a request is admitted, in
-    arrival order, once its tables are ready and its footprint fits.
-    progress, if given, is called with a message after every 1000th
-    finished request and at least every 30 s while requests finish."""
-    tables = tables or Tables()
-    if block is None:
-        block, _ = tune_block(tables, size_sample(sorted({r.L for r in requests})))
-    batch = msa.Batch(tables.luts, max_bytes=max_bytes)
-    batch.set_shape(block, epoch)
-    pool = concurrent.futures.ThreadPoolExecutor(max_workers=workers)
+def serve(requests, max_bytes, epoch=64, block=None, tables=None, keep_state=False, workers=4,
+          progress=None):
+    """Run the stream through the native request loop (bin/msa_serve, the
+    C driver of the served form); returns (per-request dicts, summary dict).
+    block=None lets the driver's autotuner pick the batch kernel's block size
+    (one request per lattice size of the stream).  Requests are admitted in
+    arrival order once their refresh tables (built by `workers` host threads)
+    are ready and their footprint fits under max_bytes.  keep_state also
+    returns each request's final spins and demons.  progress, if given, gets
+    the driver's progress lines.  `tables` is accepted for callers that hold
+    one; the driver builds the same temperature grid itself."""
+    import json
+    import os
+    import subprocess
+    import tempfile
+
+    del tables
     reqs = sorted(requests, key=lambda r: r.arrival)
-    clock = clock or time.perf_counter
-    t_start = clock()
-    nxt = 0
-    queue = []
-    active = {}                 # slot -> (request, admit time)
-    prepared = {}               # rid -> future of its job tables
-    done = {}
-    updates = 0
-    max_active = 0
-    max_bytes_used = 0
-    finishing = []              # slots that reach their length in the run enqueued last
-    last_note = [clock()]       # when progress was last called
-
-    def retire(slot):
-        nonlocal updates
-        req, t_admit = active.pop(slot)
-        fin = clock() - t_start
-        rec = {"rid": req.rid, "L": req.L, "niter": req.niter, "arrival": req.arrival,
-               "admit": t_admit, "finish": fin, "latency": fin - req.arrival,
-               "energy": batch.measure(slot)["energy"]}
+    with tempfile.TemporaryDirectory(prefix="msa_serve_") as d:
+        rq = os.path.join(d, "requests.txt")
+        with open(rq, "w") as f:
+            for r in reqs:
+                f.write("%d %d %d %d %.17g %.17g %d %.9f\n"
+                        % (r.rid, r.L, r.R, r.niter, r.T0, r.T1, r.seed, r.arrival))
+        res = os.path.join(d, "results.txt")
+        cmd = [SERVE_BIN, rq, res, "--max-bytes", str(int(max_bytes)), "--epoch", str(epoch),
+               "--block", str(block or 0), "--workers", str(workers)]
         if keep_state:
-            rec["state"] = batch.download(slot)
-        done[req.rid] = rec
-        updates += 64 * req.R * req.L ** 3 * req.niter
-        batch.remove(slot)
-        if progress and (len(done) % 1000 == 0 or clock() - last_note[0] >= 30.0):
-            last_note[0] = clock()
-            progress("%d of %d requests finished" % (len(done), len(reqs)))
-
-    try:
-        while len(done) < len(reqs):
-            now = clock() - t_start
-            while nxt < len(reqs) and reqs[nxt].arrival <= now:
-                req = reqs[nxt]
-                prepared[req.rid] = pool.submit(tables.job, req, req.rid)
-                queue.append(req)
-                nxt += 1
-            while queue:                              # FIFO admission by footprint
-                req = queue[0]
-                fut = prepared[req.rid]
-                if not fut.done() and active:
-                    break                             # admitted at a later epoch
-                slot = batch.add(fut.result())
-                if slot is None:
-                    if not active:
-                        raise RuntimeError("request %d alone exceeds the memory limit" % req.rid)
-                    break
-                queue.pop(0)
-                del prepared[req.rid]
-                active[slot] = (req, clock() - t_start)
-            max_active = max(max_active, len(active))
-            max_bytes_used = max(max_bytes_used, batch.bytes_in_use())
-            running = [s for s in active if s not in finishing]
-            if not running and not finishing:
-                if nxt < len(reqs):
-                    time.sleep(max(0.0, min(0.01, reqs[nxt].arrival - (clock() - t_start))))
-                continue
-            # one epoch stays queued on the device while the previous one's
-            # finished requests are measured and retired
-            new_finishing = []
-            if running:
-                batch.run(epoch)
-                t, tend = batch.planned(max(running) + 1)
-                new_finishing = [s for s in running if t[s] >= tend[s]]
-                batch.wait_previous()
-            else:
-                batch.sync()
-            for slot in finishing:
-                retire(slot)
-            finishing = new_finishing
-        makespan = clock() - t_start
-    finally:
-        pool.shutdown()
-        batch.close()
-    lat = np.array([done[r.rid]["latency"] for r in reqs])
-    summary = {"requests": len(reqs), "makespan_s": makespan,
-               "spin_updates_per_ps": updates / makespan / 1e12,
-               "latency_p50_s": float(np.percentile(lat, 50)),
-               "latency_p99_s": float(np.percentile(lat, 99)),
-               "max_concurrent": max_active, "max_bytes_in_use": max_bytes_used,
-               "epoch": epoch, "block": block}
+            os.makedirs(os.path.join(d, "state"))
+            cmd += ["--state-dir", os.path.join(d, "state")]
+        proc = subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+        err_lines = []
+        for ln in proc.stderr:
+            err_lines.append(ln)
+            if progress:
+                progress(ln.rstrip())
+        out = proc.stdout.read()
+        if proc.wait() != 0:
+            raise msa.MsaError("msa_serve failed: %s" % "".join(err_lines[-20:]))
+        summary = json.loads(out.strip().splitlines()[-1])
+        done = {}
+        with open(res) as f:
+            for ln in f:
+                v = ln.split()
+                rid, L, R, niter = (int(x) for x in v[:4])
+                arrival, admit, finish, latency = (float(x) for x in v[4:8])
+                rec = {"rid": rid, "L": L, "niter": niter, "arrival": arrival, "admit": admit,
+                       "finish": finish, "latency": latency,
+                       "energy": np.array(v[8:8 + 64 * R], np.int32).reshape(R, 64)}
+                if keep_state:
+                    V = L ** 3
+                    st = np.fromfile(os.path.join(d, "state", "req_%d.bin" % rid), np.uint64)
+                    rec["state"] = (st[:R * V].reshape(R, V).copy(),
+                                    st[R * V:2 * R * V].reshape(R, 1, V).copy(),
+                                    st[2 * R * V:].reshape(R, 1, V).copy())
+                done[rid] = rec
     return done, summary

@@ -17,6 +17,9 @@ pytestmark = pytest.mark.gpu
 
 def modules():
     import importlib
+
+    from conftest import load_package
+    load_package()
     return (importlib.import_module("micsa_ea_mc_amd.serve"),
             importlib.import_module("micsa_ea_mc_amd.msa"))
 
