@@ -240,37 +240,57 @@ def test_restart_after_maxtime_continues_bit_for_bit(tmp_path, nterm, sched):
 LONG = dict(L=32, nr=2, nc=1, maxtime=2 ** 20, times=tuple(2 ** k for k in range(21)))
 
 
+def long_args(d, mode):
+    """No schedule file: the reference's own workload (walker streams);
+    'philox': the Philox refresh at beta.dat, same cadence (t = 0, 1, 2, 4, ...)."""
+    if mode == "reference":
+        return []
+    return fixed_schedule(d, LONG["maxtime"])
+
+
 @pytest.fixture(scope="module")
-def long_reference(tmp_path_factory):
-    d = str(tmp_path_factory.mktemp("long_ref"))
-    out = write_inputs(d, **LONG)
-    t0 = time.time()
-    p = run_driver(d)
-    assert p.returncode == 0, p.stderr
-    print("uninterrupted run: %.1f s" % (time.time() - t0), flush=True)
-    return out
+def long_uninterrupted(tmp_path_factory):
+    """The uninterrupted runs, one per refresh mode, made when first asked for."""
+    runs = {}
+
+    def get(mode):
+        if mode not in runs:
+            d = str(tmp_path_factory.mktemp("long_" + mode))
+            out = write_inputs(d, **LONG)
+            t0 = time.time()
+            p = run_driver(d, long_args(d, mode))
+            assert p.returncode == 0, p.stderr
+            print("uninterrupted %s run: %.1f s" % (mode, time.time() - t0), flush=True)
+            runs[mode] = out
+        return runs[mode]
+    return get
 
 
 @pytest.mark.gpu
-def test_wall_budget_then_restart(tmp_path, long_reference):
+@pytest.mark.parametrize("mode", ["reference", "philox"])
+def test_wall_budget_then_restart(tmp_path, long_uninterrupted, mode):
     d = str(tmp_path)
     out = write_inputs(d, **LONG)
-    p = run_driver(d, ["1"])                          # max_time = 1 s
+    extra = long_args(d, mode)
+    p = run_driver(d, ["1"] + extra[1:] if extra else ["1"])     # max_time = 1 s
     assert p.returncode == 0, p.stderr
     stop = int(p.stdout.split("finished at iteration ")[1].split()[0])
     assert stop < LONG["maxtime"] and stop in LONG["times"], stop
     write_inputs(d, flag=2, **LONG)
-    p = run_driver(d)
+    p = run_driver(d, extra)
     assert p.returncode == 0 and "finished at iteration %d" % LONG["maxtime"] in p.stdout
-    assert_same_outputs(long_reference, out, (0, 1), LONG["nr"])
+    assert_same_outputs(long_uninterrupted(mode), out, (0, 1), LONG["nr"])
 
 
 @pytest.mark.gpu
-def test_sigterm_then_restart(tmp_path, long_reference):
+@pytest.mark.parametrize("mode", ["reference", "philox"])
+def test_sigterm_then_restart(tmp_path, long_uninterrupted, mode):
     d = str(tmp_path)
     out = write_inputs(d, **LONG)
+    extra = long_args(d, mode)
     args = [DRIVER, "3", "2", os.path.join(d, "beta.in"), os.path.join(d, "input.in"),
-            os.path.join(d, "lut.bin"), os.path.join(d, "times.in"), "0"]
+            os.path.join(d, "lut.bin"), os.path.join(d, "times.in"), "0"] + extra
+    ref = long_uninterrupted(mode)
     proc = subprocess.Popen(args, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
     mark = os.path.join(sample_dir(out), "BIT000", "R00", "conf.%016d" % 1024)
     t0 = time.time()
@@ -284,9 +304,9 @@ def test_sigterm_then_restart(tmp_path, long_reference):
     assert "Received SIGTERM" in open(os.path.join(sd, "error")).read()
     assert not os.path.exists(os.path.join(sd, "running"))
     write_inputs(d, flag=2, **LONG)
-    p = run_driver(d)
+    p = run_driver(d, extra)
     assert p.returncode == 0 and "finished at iteration %d" % LONG["maxtime"] in p.stdout, p.stderr
-    assert_same_outputs(long_reference, out, (0, 1), LONG["nr"])
+    assert_same_outputs(ref, out, (0, 1), LONG["nr"])
 
 
 @pytest.mark.gpu
