@@ -154,6 +154,59 @@ def cpu_baseline(L, R, nc, budget_s, threads, every, measure_every):
             "nproc": cpu["nproc"], "cpu_affinity": cpu["affinity"], "cpu_model": cpu["model"]}
 
 
+def cpu_extras(threads, budget_s=3.0):
+    """The rest of the CPU-baseline plan, on this host with the oracle:
+    the batched configuration's serial loop (each lattice size of the mix
+    {16,24,32,48,64,96}, R=1, refresh every sweep, timed for about budget_s
+    and weighted by the mix's volume: 512 lattices, lengths from
+    {4096,16384,65536}, xoshiro256++ seed 4242), and the reference's own demon
+    refresh (InitWalkers, ini.c:712-781: one sequential PR+LCG and
+    xoshiro256++ stream per sample and replica; the reference runs one host
+    thread per sample, main.cu:371-375) as demon words per second and the
+    cost of one refresh of the reference's workload (L=160, nc=6, 16
+    replicas per sample)."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+    from sched_util import oracle_schedule
+    orc = oracle_lib.load()
+    rng = Xoshiro256pp(4242)
+    sizes = (16, 24, 32, 48, 64, 96)
+    lens = (4096, 16384, 65536)
+    lats = [(sizes[rng.next() % 6], lens[rng.next() % 3]) for _ in range(512)]
+    per = {}
+    for L in sizes:
+        o = oracle_lib.Lattice(L, 1, 1)
+        orc.init_philox(o, 77)
+        ctr, keys, _, luts = oracle_schedule(orc, 1, [0], [1.0 / 1.5])
+        n, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < budget_s / len(sizes) or n < 2:
+            orc.iteration(o, ctr[0], keys, luts[0], nthreads=threads)
+            n += 1
+        per[L] = (time.perf_counter() - t0) / n
+    serial_s = sum(per[L] * n for L, n in lats)
+    updates = sum(64.0 * L ** 3 * n for L, n in lats)
+    pr, xo, _ = orc.cpu_generators(2002)
+    thr = orc.thresholds(0.90909090909090)
+    nwords = 1 << 16
+    t0 = time.perf_counter()
+    orc.init_walkers(pr, xo, thr, nwords)
+    wps = nwords / (time.perf_counter() - t0)
+    ref_words = 160 ** 3 // 64 * 6 * 16
+    return {"batched_serial_loop": {
+                "spin_updates_per_ps": round(updates / serial_s / 1e12, 6),
+                "serial_s_for_the_mix": round(serial_s, 1), "threads": threads,
+                "sweep_ms_by_L": {str(L): round(per[L] * 1e3, 4) for L in sizes},
+                "note": "per-size oracle refresh sweeps timed on this host and weighted by "
+                        "the 512-lattice mix: the loop itself would run serial_s_for_the_mix"},
+            "reference_demon_refresh": {
+                "demon_words_per_s_per_thread": round(wps),
+                "s_per_refresh_of_one_sample": round(ref_words / wps, 3),
+                "note": "InitWalkers restatement (oracle), one stream: 64 draws per word; one "
+                        "refresh of one sample of the reference's workload (L=160, nc=6, 16 "
+                        "replicas) is %d words on one host thread" % ref_words}}
+
+
 def nref_window(t0, steps, every):
     """Refreshes among iterations t0 .. t0+steps-1 at a uniform cadence."""
     if every <= 0:
@@ -349,6 +402,9 @@ def main():
                     help="oracle threads (default: the cores this process may use, at most 16, "
                          "the host share of one GPU)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-extras", action="store_true",
+                    help="also time the batched mix's serial CPU loop and the reference's host "
+                         "demon refresh (InitWalkers) with the oracle")
     ap.add_argument("--batched", type=int, default=512, help="lattices in the batched test (0: skip)")
     ap.add_argument("--batched-len-div", type=int, default=16,
                     help="anneal lengths {4096,16384,65536} divided by this")
@@ -567,6 +623,9 @@ def main():
     if not args.no_cpu:
         progress("CPU baseline: %.0f s" % args.cpu_budget)
         out["cpu_baseline"] = cpu_baseline(L, R, nc, args.cpu_budget, args.cpu_threads, every, me)
+        if args.cpu_extras:
+            progress("CPU extras: batched serial loop, reference demon refresh")
+            out["cpu_baseline"]["extras"] = cpu_extras(args.cpu_threads)
     print(json.dumps(out))
 
 
