@@ -240,6 +240,12 @@ static double percentile(double *v, int n, double p)
   return v[lo] + (v[hi] - v[lo]) * (pos - lo);
 }
 
+static int cmp_int(const void *a, const void *b)
+{
+  const int x = *(const int *)a, y = *(const int *)b;
+  return x < y ? -1 : x > y;
+}
+
 static int cmp_arrival(const void *a, const void *b)
 {
   const request *x = (const request *)a, *y = (const request *)b;
@@ -298,6 +304,16 @@ int main(int argc, char **argv)
   fclose(f);
   if (!reqs) fail("out of memory");
   if (n == 0) fail("no requests in %s", argv[1]);
+  {
+    /* request ids name the results and state files: they must be unique */
+    int *ids = (int *)malloc(sizeof(int) * (size_t)n);
+    if (!ids) fail("out of memory");
+    for (int i = 0; i < n; i++) ids[i] = reqs[i].rid;
+    qsort(ids, (size_t)n, sizeof *ids, cmp_int);
+    for (int i = 0; i + 1 < n; i++)
+      if (ids[i] == ids[i + 1]) fail("request id %d appears twice", ids[i]);
+    free(ids);
+  }
   qsort(reqs, (size_t)n, sizeof *reqs, cmp_arrival);
   for (int i = 0; i < n; i++) {
     reqs[i].energy = (int32_t *)calloc((size_t)reqs[i].R * 64, sizeof(int32_t));

@@ -24,7 +24,8 @@ def test_bad_request_lines(tmp_path):
                       ("# only a comment\n", "no requests"),
                       ("0 15 1 64 2.0 0.7 7 0.0\n", "L=15"),
                       ("0 16 17 64 2.0 0.7 7 0.0\n", "R=17"),
-                      ("0 16 1 0 2.0 0.7 7 0.0\n", "niter=0")):
+                      ("0 16 1 0 2.0 0.7 7 0.0\n", "niter=0"),
+                      ("3 16 1 64 2.0 0.7 7 0.0\n3 16 1 64 2.0 0.7 8 0.1\n", "appears twice")):
         open(rq, "w").write(text)
         p = run([rq, str(tmp_path / "out.txt")])
         assert p.returncode == 1 and msg in p.stderr, (text, p.stderr)
