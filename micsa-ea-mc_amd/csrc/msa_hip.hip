@@ -1646,7 +1646,10 @@ extern "C" int msa_measure(msa_ctx *c, int32_t *unsat, int32_t *demon, int32_t *
   a.jp = c->jp;
   a.dm = c->dm[c->cur];
   a.g = g;
-  a.nr = g.Lh < 512 ? 512 / g.Lh : 1;   // blocks of at most 512 threads
+  // blocks of about 256 threads (at most 512): more, smaller blocks per CU
+  // hide the latency of the row loads better (L = 160: 103 instead of 131 us
+  // per call with 480-thread blocks)
+  a.nr = g.Lh < 256 ? 256 / g.Lh : 1;
   a.mLh = div_recip_of(g.Lh);
   a.out = c->meas;
   // rows per thread: as many as leave one block per CU and job (fewer
