@@ -75,6 +75,13 @@ __device__ __forceinline__ void sto_stream(ulonglong2 *base, uint32_t boff, uint
   __builtin_nontemporal_store(v, reinterpret_cast<u64x2 *>(reinterpret_cast<char *>(base) + boff));
 }
 
+// s_waitcnt vmcnt(0) (expcnt and lgkmcnt at their maxima, gfx9 encoding):
+// every vector memory access this wave issued has completed.  Placed where
+// the loads are needed anyway, it lets the compiler's wait counting treat
+// them as done, so that loads issued after it for a later step are the only
+// accesses in flight and later waits need not cover them.
+__device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // ---------------------------------------------------------------- Philox --
 // Philox4x32-10 with the round and key schedule of main.cu:562-595
 // (Random123's philox4x32 with 10 rounds).

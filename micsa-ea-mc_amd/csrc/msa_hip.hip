@@ -596,6 +596,12 @@ __device__ __forceinline__ void tile_of_pair(uint32_t b, uint32_t nty, uint32_t 
 // is one the same thread loaded two to four steps earlier, kept in register
 // windows.  The spin and result planes are double-buffered, so one barrier
 // per step orders all writers and readers.
+// A step's own colour-0 spin, halo spin and demons are loaded during the step
+// before, right after the phase that last used their registers, so they are
+// in flight through three phases and the barrier; the other loads are issued
+// together before the barrier.  The demons are read with ordinary loads (a
+// halo row's demons are read again by the neighbouring tile, which runs on the
+// same XCD) and the results are written with the non-temporal hint.
 __global__ void __launch_bounds__(1024) k_sweep_pair(PairArgs a) {
   using msa::ldo;
   using msa::ldo_stream;
@@ -633,6 +639,10 @@ __global__ void __launch_bounds__(1024) k_sweep_pair(PairArgs a) {
   uint64_t Sm = ldo(S1in, (i + yb + LhL * wrapi((int)z0 - 4, L)) * 8u);
   uint64_t S0w = ldo(S1in, (i + yb + LhL * wrapi((int)z0 - 3, L)) * 8u);
   uint64_t Sp = 0;
+  // loaded one step ahead, into the registers the step before has just
+  // finished with: A0's own spin, its halo spin and A0's and A1's demons
+  uint64_t S0v = ldo(S0in, (i + yb + LhL * wrapi((int)z0 - 3, L)) * 8u);
+  uint64_t Hv = hrow ? ldo(S1in, (i + hrb + LhL * wrapi((int)z0 - 3, L)) * 8u) : 0;
   uint64_t E0 = 0, F0 = 0, G0 = 0;        // A0, A1, B0 results of the step (planes s, s-1, s-2)
   // J windows, [d] = loaded d+1 steps ago: the first sweep's own bonds
   // (colour 0 at plane s-1-d: x, y, z; colour 1: x, y, z), the colour-1 bonds
@@ -641,12 +651,17 @@ __global__ void __launch_bounds__(1024) k_sweep_pair(PairArgs a) {
   uint64_t w1x[3] = {0, 0, 0}, w1y[3] = {0, 0, 0}, w1z[3] = {0, 0, 0};
   uint64_t wc0x[2] = {0, 0}, wc0y[2] = {0, 0}, wc1x[2] = {0, 0}, wc1y[2] = {0, 0};
   w1z[0] = ldo(Jz1, (i + yb + LhL * wrapi((int)z0 - 4, L)) * 8u);
+  ulonglong2 dA0 = ldo(Din0, (i + yb + LhL * wrapi((int)z0 - 3, L)) * 16u);
+  ulonglong2 dA1 = make_ulonglong2(0, 0);
+  // the prime loads complete before the loop, so that the loop's only wait
+  // is the one after each barrier
+  msa::wait_vmem();
 
   const int nstep = (int)a.tz + 6;
   for (int k = 0; k < nstep; ++k) {
     const int s = (int)z0 - 3 + k;
     const uint32_t z = wrapi(s, L), z1 = wrapi(s - 1, L), z2 = wrapi(s - 2, L), z3 = wrapi(s - 3, L);
-    const uint32_t h = i + yb + LhL * z, h1 = i + yb + LhL * z1;
+    const uint32_t h = i + yb + LhL * z;
     const uint32_t h2 = i + yb + LhL * z2, h3 = i + yb + LhL * z3;
     const uint32_t off = (y + z) & 1u;
     const uint32_t ip = off ? (i + 1 == Lh ? 0 : i + 1) : i, im = off ? i : (i == 0 ? Lh - 1 : i - 1);
@@ -658,16 +673,13 @@ __global__ void __launch_bounds__(1024) k_sweep_pair(PairArgs a) {
     const uint64_t j0x = ldo(Jx0, h * 8u), j0y = ldo(Jy0, h * 8u), j0z = ldo(Jz0, h * 8u);
     const uint64_t j1x = ldo(Jx1, h * 8u), j1y = ldo(Jy1, h * 8u), j1z = ldo(Jz1, h * 8u);
     const uint64_t c0J3 = ldo(Jx1, (im + yb + LhL * z) * 8u), c0J4 = ldo(Jy1, (i + ymb + LhL * z) * 8u);
-    const uint64_t S0v = ldo(S0in, h * 8u);
-    const ulonglong2 dA0 = ldo_stream(Din0, h * 16u);
-    const uint64_t Hv = hrow ? ldo(S1in, (i + hrb + LhL * z) * 8u) : 0;
     // A1's loads (its -x and -y colour-0 bonds, its demons) with the others
     const bool doA1 = rowA1 && k >= 2;
     const uint64_t c1J3 = doA1 ? ldo(Jx0, (im + yb + LhL * z1) * 8u) : 0;
     const uint64_t c1J4 = doA1 ? ldo(Jy0, (i + ymb + LhL * z1) * 8u) : 0;
-    const ulonglong2 dA1 = doA1 ? ldo_stream(Din1, h1 * 16u) : make_ulonglong2(0, 0);
     S1p[cur * P + slot] = S0w;
     __syncthreads();
+    msa::wait_vmem();
     // ---- A0: first sweep, colour 0, plane s
     {
       uint64_t D0 = dA0.x, D1 = dA0.y;
@@ -680,6 +692,13 @@ __global__ void __launch_bounds__(1024) k_sweep_pair(PairArgs a) {
       A0b[ring * P + slot] = E0;
       XD0[ring * P + row * Lh + iw0] = make_ulonglong2(D0, D1);   // walked, read by B0 at k+2
     }
+    // the next step's A0 inputs, in flight through A1, B0, B1 and the barrier
+    if (k + 1 < nstep) {
+      const uint32_t zn = wrapi(s + 1, L);
+      dA0 = ldo(Din0, (i + yb + LhL * zn) * 16u);
+      S0v = ldo(S0in, (i + yb + LhL * zn) * 8u);
+      Hv = hrow ? ldo(S1in, (i + hrb + LhL * zn) * 8u) : 0;
+    }
     // ---- A1: first sweep, colour 1, plane s-1
     if (doA1) {
       uint64_t D0 = dA1.x, D1 = dA1.y;
@@ -691,6 +710,8 @@ __global__ void __launch_bounds__(1024) k_sweep_pair(PairArgs a) {
       A1b[ring * P + slot] = F0;
       XD1[ring * P + row * Lh + iw1] = make_ulonglong2(D0, D1);   // walked, read by B1 at k+2
     }
+    if (rowA1 && k + 1 >= 2 && k + 1 < nstep) dA1 = ldo(Din1, h * 16u);   // next step's A1
+
     // ---- B0: second sweep, colour 0, plane s-2 (J of A0 two steps ago)
     if (rowB0 && k >= 4) {
       const ulonglong2 d = XD0[ring2 * P + slot];
