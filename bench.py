@@ -13,8 +13,10 @@ clock (the sweeps alone, timed with HIP events on the library's stream, are
 reported beside it).  The configuration's work mix is one refresh sweep and
 one measurement per 1024 sweeps: when the window holds fewer of them than
 its length calls for (a window shorter than 1024 sweeps holds none), the
-extra cost of a refresh block and of a measurement, timed separately after
-the window, is added to the window's time for the share it lacks.
+extra cost of a refresh (a refresh sweep and one sweep against one sweep
+pair, median of three short back-to-back blocks) and of a measurement, timed
+separately after the window, is added to the window's time for the share it
+lacks.
 
 The sweep kernel and its tile are first autotuned on this lattice (every
 candidate gives bit-identical results; --no-autotune keeps the default).
@@ -443,9 +445,9 @@ def main():
     tiling = lat.tiling()
     total = args.warmup + args.steps
     sched = pkg.schedule(aT0, aT1, nsched, every)
-    # tables reach two refreshes past the timed window: the refresh block
-    # timed after it (below) starts at the first refresh after the window
-    ntab = total + 2 * every if every > 0 else total
+    # tables reach four refreshes past the timed window: the refresh blocks
+    # timed after it (below) start at the first refreshes after the window
+    ntab = total + 4 * every if every > 0 else total
     ctr, keys, luts, idx = pkg.schedule_tables(sched, ntab, R, [3003 + r for r in range(R)], 0)
     lat.set_luts(luts)
     lat.set_refresh(ctr, keys, idx)
@@ -475,8 +477,22 @@ def main():
     # window lacks is added to its time (value = updates / (wall + missing))
     t_after = args.warmup + args.steps
     t_ref_extra = 0.0
-    if every > 0:
+    if every >= 4:
+        # a refresh makes a block of `every` sweeps run the refresh sweep, one
+        # single sweep and the rest in pairs where `every` plain sweeps run
+        # all in pairs: its extra cost is (refresh sweep + one sweep) - (one
+        # pair).  Timed on short back-to-back blocks, median of three, so that
+        # clock drift between two long blocks does not enter the difference.
         tb = (t_after // every + 1) * every          # first refresh after the window
+        diffs = []
+        for j in range(3):
+            t0r = tb + j * every
+            ms_ref = lat.run_timed(t0r, 2, every)        # refresh sweep + one sweep
+            ms_plain = lat.run_timed(t0r + 2, 2, every)  # one pair
+            diffs.append(ms_ref - ms_plain)
+        t_ref_extra = max(0.0, float(np.median(diffs)) * 1e-3)
+    elif every > 0:
+        tb = (t_after // every + 1) * every
         ms_ref = lat.run_timed(tb, every, every)     # `every` sweeps, the first one refreshing
         ms_plain = lat.run_timed(tb + every, every, 0)
         t_ref_extra = max(0.0, (ms_ref - ms_plain) * 1e-3)
