@@ -240,11 +240,24 @@ def test_restart_after_maxtime_continues_bit_for_bit(tmp_path, nterm, sched):
 LONG = dict(L=32, nr=2, nc=1, maxtime=2 ** 20, times=tuple(2 ** k for k in range(21)))
 
 
+def annealed_schedule(d, nsweeps, every=256):
+    """Arguments selecting an annealing run: T 2.0 -> 0.7 over nsweeps sweeps,
+    Philox refresh every `every` sweeps, one LUT pair per refresh."""
+    path = os.path.join(d, "sched.in")
+    with open(path, "w") as f:
+        f.write("2.0\n0.7\n%d\n%d\n" % (nsweeps, every))
+    return ["0", "", path]
+
+
 def long_args(d, mode):
     """No schedule file: the reference's own workload (walker streams);
-    'philox': the Philox refresh at beta.dat, same cadence (t = 0, 1, 2, 4, ...)."""
+    'philox': the Philox refresh at beta.dat, same cadence (t = 0, 1, 2, 4, ...);
+    'annealed': a schedule file, T 2.0 -> 0.7 over the whole run, refresh
+    every 256 sweeps (a restart resumes the anneal at its refresh index)."""
     if mode == "reference":
         return []
+    if mode == "annealed":
+        return annealed_schedule(d, LONG["maxtime"])
     return fixed_schedule(d, LONG["maxtime"])
 
 
@@ -267,7 +280,7 @@ def long_uninterrupted(tmp_path_factory):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("mode", ["reference", "philox"])
+@pytest.mark.parametrize("mode", ["reference", "philox", "annealed"])
 def test_wall_budget_then_restart(tmp_path, long_uninterrupted, mode):
     d = str(tmp_path)
     out = write_inputs(d, **LONG)
@@ -283,7 +296,7 @@ def test_wall_budget_then_restart(tmp_path, long_uninterrupted, mode):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("mode", ["reference", "philox"])
+@pytest.mark.parametrize("mode", ["reference", "philox", "annealed"])
 def test_sigterm_then_restart(tmp_path, long_uninterrupted, mode):
     d = str(tmp_path)
     out = write_inputs(d, **LONG)
