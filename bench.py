@@ -423,6 +423,8 @@ def main():
                     help="also time a streaming copy of this many MiB (bandwidth calibration)")
     ap.add_argument("--tiling", type=str, default="",
                     help="MODE,TY,TZ: use this sweep kernel and tile (no autotune)")
+    ap.add_argument("--refresh-mode", type=int, default=0,
+                    help="refresh sweeps: 0 drawn inside the sweep, 1 drawn ahead on a second stream")
     ap.add_argument("--thermal-iters", type=int, default=0,
                     help="also time this many Metropolis thermalization iterations (msa_run_thermal)")
     args = ap.parse_args()
@@ -443,6 +445,7 @@ def main():
     elif not args.no_autotune:
         tuned = lat.autotune(16)
     tiling = lat.tiling()
+    lat.set_refresh_mode(args.refresh_mode)
     total = args.warmup + args.steps
     sched = pkg.schedule(aT0, aT1, nsched, every)
     # tables reach four refreshes past the timed window: the refresh blocks

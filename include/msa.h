@@ -189,6 +189,19 @@ int msa_get_walker_streams(msa_ctx *ctx, msa_walker_stream *st);
 int msa_set_tiling(msa_ctx *ctx, int32_t mode, int32_t ty, int32_t tz);
 int msa_get_tiling(msa_ctx *ctx, int32_t *mode, int32_t *ty, int32_t *tz);
 
+/* How msa_run performs the sweeps with a Philox refresh (no reference
+ * counterpart; the reference draws its refresh on the host, main.cu:340-377).
+ * mode 0: the refresh is drawn inside the sweep's demon load (two kernels per
+ * sweep, one per colour).  mode 1: the demons of the coming refreshes are
+ * drawn ahead, up to 64 refreshes per launch, by a separate kernel on a
+ * second stream of the context (it overlaps the sweeps before them; at most
+ * 512 MiB of device memory, allocated on the first such run), and each
+ * refresh sweep runs the refresh-free kernels of msa_set_tiling reading them.
+ * Results are identical in both modes.  Walker-stream refreshes are not
+ * affected. */
+int msa_set_refresh_mode(msa_ctx *ctx, int32_t mode);
+int msa_get_refresh_mode(msa_ctx *ctx, int32_t *mode);
+
 /* Autotuner (no reference counterpart): times `sweeps` sweeps of every
  * kernel choice valid for this geometry on a scratch lattice of the same
  * geometry (the context's own state is not touched), applies the fastest and

@@ -34,6 +34,15 @@ struct msa_ctx {
   int fyhalo = 1;
   int pair = 0;                           // refresh-free sweeps two per pass (k_sweep_pair)
   uint32_t pty = 0, ptz = 0;              // its tile: rows, planes
+  // Philox refresh sweeps: 0 = the refresh fused into the two-pass kernel's
+  // demon load; 1 = demons drawn ahead by k_refresh_draw on a second stream
+  // and read by the refresh-free kernels
+  int predraw = 0;
+  hipStream_t dstream = nullptr;
+  hipEvent_t e_drawn[2] = {nullptr, nullptr}, e_used[2] = {nullptr, nullptr};
+  ulonglong2 *pre = nullptr;              // [2 pK][R nc V] drawn demons (two halves)
+  uint32_t pK = 0;                        // refreshes per half
+  uint32_t ncu = 0;                       // compute units of the device
   // demon refresh from the reference's walker streams (msa_walkers.hip)
   int32_t w_n = 0;                        // streams per replica, 0 = Philox refresh
   int w_cur = 0;                          // w_st buffer holding the current states

@@ -170,6 +170,66 @@ __global__ void __launch_bounds__(1024) k_halfsweep(SweepArgs a) {
   }
 }
 
+// Refresh demons drawn ahead of their sweeps: the words k_halfsweep<kRefresh>
+// draws in its demon load, written out as a demon buffer so that a sweep
+// without a refresh reading it as its input demons performs the refresh
+// sweep.  Block (bx, j) draws refresh k0 + j into out + j * R nc V: it stages
+// that refresh's two search tables in LDS once, then walks the planes
+// p = (r nc + l) 2 + c (wave-uniform: replica r, layer l, colour c) and, in
+// each, rows of L/2 sites (thread = nr rows x L/2 sites, a grid-stride loop
+// over the L^2 rows); the site's thread id is tid = l V + x + L (y + L z), as
+// in the fused refresh.  Pure VALU work (Philox and the table search) and one
+// streaming store per word: run on a second stream, it fills the issue slots
+// the memory-bound sweeps leave idle.
+constexpr int kDrawMax = 64;   // refreshes per draw launch (DrawArgs::lidx)
+// dynamic LDS of a draw block: the two search tables, requested as more than
+// half of a CU's 160 KiB so that a CU holds one draw block
+constexpr size_t kDrawLds = 84 * 1024;
+struct DrawArgs {
+  ulonglong2 *out;              // [nk][R nc 2 H]
+  const uint4 *ctr;             // counters of refresh k0: ctr[j R + r] is refresh k0 + j
+  const uint32_t *key;          // [R]
+  const msa::FineLut *luts;     // search tables, pair n at luts + 2 n
+  Geo g;
+  uint32_t nr;                  // rows per block
+  uint32_t mLh, mL;             // div_recip reciprocals of L/2 and L
+  int32_t lidx[kDrawMax];       // LUT pair of each refresh of the launch
+};
+
+__global__ void __launch_bounds__(768) k_refresh_draw(DrawArgs a) {
+  extern __shared__ uint4 s_tab[];
+  const Geo g = a.g;
+  const uint32_t j = blockIdx.y;
+  {
+    const uint4 *src = reinterpret_cast<const uint4 *>(a.luts + 2 * (size_t)a.lidx[j]);
+    const uint32_t n16 = 2 * sizeof(msa::FineLut) / 16;
+    for (uint32_t k = threadIdx.x; k < n16; k += blockDim.x) s_tab[k] = src[k];
+    __syncthreads();
+  }
+  const msa::FineLut *tabs = reinterpret_cast<const msa::FineLut *>(s_tab);
+  const msa::LutView t0 = msa::lut_view(tabs), t1 = msa::lut_view(tabs + 1);
+  const uint32_t L = g.L, Lh = g.Lh, nrows = L * L;
+  const uint32_t rloc = div_recip(threadIdx.x, a.mLh), i = threadIdx.x - rloc * Lh;
+  if (rloc >= a.nr) return;
+  const uint32_t np = 2 * g.R * g.nc;
+  ulonglong2 *out = a.out + (uint64_t)j * np * g.H;
+  for (uint32_t p = 0; p < np; ++p) {
+    const uint32_t col = p & 1u, l = (p >> 1) % g.nc, r = (p >> 1) / g.nc;
+    const uint4 c4 = a.ctr[(size_t)j * g.R + r];
+    const uint32_t kr = a.key[r];
+    ulonglong2 *o = out + (uint64_t)p * g.H;
+    for (uint32_t row = blockIdx.x * a.nr + rloc; row < nrows; row += gridDim.x * a.nr) {
+      const uint32_t z = div_recip(row, a.mL), y = row - z * L;
+      const uint32_t x = 2u * i + ((y + z + col) & 1u);
+      const uint32_t tid = (uint32_t)((uint64_t)l * g.V + x + (uint64_t)L * row);
+      const uint64_t D0 = msa::refresh_plane(t0, c4, kr, tid, 0);
+      const uint64_t D1 = msa::refresh_plane(t1, c4, kr, tid, 1);
+      const msa::u64x2 v = {D0, D1};
+      __builtin_nontemporal_store(v, reinterpret_cast<msa::u64x2 *>(o + i + (uint64_t)Lh * row));
+    }
+  }
+}
+
 // Search tables of heat-bath tables first .. ntab-1 (msa::FineLut), table
 // first + blockIdx.y: thresholds copied, presearch entry b = the byte of the
 // first R of bucket b, found by binary search for the smallest n with
@@ -270,6 +330,7 @@ struct FusedArgs {
   uint32_t ty, tz;   // rows per tile, planes per slab
   uint32_t ny;       // thread rows per block: ty + 2 (halo rows) or L (whole planes)
   int yhalo;
+  int dm_store;      // 0: the walked demons are not written (the next sweep refreshes them)
 };
 
 // A whole sweep (both colours, one demon layer, walk fused) in one pass.
@@ -418,6 +479,9 @@ __global__ void __launch_bounds__(R >= 4 ? 768 : 1024) k_sweep_fused2(FusedArgs 
   using msa::ldo_stream;
   using msa::sto_stream;
   extern __shared__ uint64_t s_lds[];
+  // ahead of the refresh draws that may share the CU (they fill the issue
+  // slots this latency-bound kernel leaves)
+  __builtin_amdgcn_s_setprio(2);
   const Geo g = a.g;
   const uint32_t Lh = g.Lh, L = g.L;
   const uint64_t H = g.H;
@@ -509,7 +573,7 @@ __global__ void __launch_bounds__(R >= 4 ? 768 : 1024) k_sweep_fused2(FusedArgs 
         Ew[r * pr + slot] = E0[r];
         if (write) {
           sto_stream(a.sp_out + (uint64_t)(2 * r) * H, h * 8u, E0[r]);
-          sto_stream(Dout0 + r * rstride, hw * 16u, D0, D1);
+          if (a.dm_store) sto_stream(Dout0 + r * rstride, hw * 16u, D0, D1);
         }
       }
     }
@@ -533,7 +597,7 @@ __global__ void __launch_bounds__(R >= 4 ? 768 : 1024) k_sweep_fused2(FusedArgs 
         const uint64_t sn = msa::site_update(Sm[r], nxp, nyp, E0[r], nxm, nym, Em2[r], j1x_prev,
                                              j1y_prev, j1z_prev, c1J3, c1J4, j0z_m2, D0, D1);
         sto_stream(a.sp_out + (uint64_t)(2 * r + 1) * H, ho * 8u, sn);
-        sto_stream(Dout1 + r * rstride, hw * 16u, D0, D1);
+        if (a.dm_store) sto_stream(Dout1 + r * rstride, hw * 16u, D0, D1);
       }
     }
     // ---- rotate the windows
@@ -557,10 +621,12 @@ struct PairArgs {
   uint64_t *sp_out;
   const uint64_t *jp;
   const ulonglong2 *dm_in;
+  const ulonglong2 *dm_in2;   // D2: the second iteration's demons (drawn for its refresh)
   ulonglong2 *dm_out;
   Geo g;
   uint32_t ty, tz;   // own rows, own planes
   uint32_t ny;       // thread rows: ty + 6
+  int dm_store;      // 0: the walked demons are not written (the next sweep refreshes them)
 };
 
 // Block b of the sweep-pair kernel -> (row tile, plane slab, replica).  XCD x
@@ -602,12 +668,22 @@ __device__ __forceinline__ void tile_of_pair(uint32_t b, uint32_t nty, uint32_t 
 // together before the barrier.  The demons are read with ordinary loads (a
 // halo row's demons are read again by the neighbouring tile, which runs on the
 // same XCD) and the results are written with the non-temporal hint.
-__global__ void __launch_bounds__(1024) k_sweep_pair(PairArgs a) {
+// D2: the second iteration refreshes its demons -- they come from dm_in2 (the
+// refresh's demons drawn ahead) instead of the first iteration's walked ones,
+// each loaded during the step before the one that uses it; the walked-demon
+// rings are not needed (88 instead of 184 bytes of LDS per thread) and blocks
+// are at most 512 threads, so the kernel has room for the two more loads in
+// flight.
+template <bool D2>
+__global__ void __launch_bounds__(D2 ? 512 : 1024) k_sweep_pair(PairArgs a) {
   using msa::ldo;
   using msa::ldo_stream;
   using msa::sto_stream;
   using msa::wrapi;
   extern __shared__ uint64_t s_lds[];
+  // ahead of the refresh draws that may share the CU (they fill the issue
+  // slots this latency-bound kernel leaves)
+  __builtin_amdgcn_s_setprio(2);
   const Geo g = a.g;
   const uint32_t Lh = g.Lh, L = g.L, ny = a.ny;
   const uint64_t H = g.H;
@@ -629,6 +705,8 @@ __global__ void __launch_bounds__(1024) k_sweep_pair(PairArgs a) {
   uint64_t *S0out = a.sp_out + (uint64_t)(2 * r) * H, *S1out = S0out + H;
   const ulonglong2 *Din0 = a.dm_in + (uint64_t)(2 * r) * H, *Din1 = Din0 + H;
   ulonglong2 *Dout0 = a.dm_out + (uint64_t)(2 * r) * H, *Dout1 = Dout0 + H;
+  const ulonglong2 *Din20 = D2 ? a.dm_in2 + (uint64_t)(2 * r) * H : nullptr;
+  const ulonglong2 *Din21 = D2 ? Din20 + H : nullptr;
   const uint64_t *Jx0 = a.jp, *Jy0 = a.jp + H, *Jz0 = a.jp + 2 * H;
   const uint64_t *Jx1 = a.jp + 3 * H, *Jy1 = a.jp + 4 * H, *Jz1 = a.jp + 5 * H;
   const uint32_t LhL = Lh * L;
@@ -653,6 +731,7 @@ __global__ void __launch_bounds__(1024) k_sweep_pair(PairArgs a) {
   w1z[0] = ldo(Jz1, (i + yb + LhL * wrapi((int)z0 - 4, L)) * 8u);
   ulonglong2 dA0 = ldo(Din0, (i + yb + LhL * wrapi((int)z0 - 3, L)) * 16u);
   ulonglong2 dA1 = make_ulonglong2(0, 0);
+  ulonglong2 dB0 = make_ulonglong2(0, 0), dB1 = make_ulonglong2(0, 0);   // D2
   // the prime loads complete before the loop, so that the loop's only wait
   // is the one after each barrier
   msa::wait_vmem();
@@ -690,7 +769,7 @@ __global__ void __launch_bounds__(1024) k_sweep_pair(PairArgs a) {
       E0 = msa::site_update(S0v, nxp, nyp, Sp, nxm, nym, Sm, j0x, j0y, j0z, c0J3, c0J4, w1z[0],
                             D0, D1);
       A0b[ring * P + slot] = E0;
-      XD0[ring * P + row * Lh + iw0] = make_ulonglong2(D0, D1);   // walked, read by B0 at k+2
+      if (!D2) XD0[ring * P + row * Lh + iw0] = make_ulonglong2(D0, D1);   // walked, read by B0 at k+2
     }
     // the next step's A0 inputs, in flight through A1, B0, B1 and the barrier
     if (k + 1 < nstep) {
@@ -708,13 +787,13 @@ __global__ void __launch_bounds__(1024) k_sweep_pair(PairArgs a) {
                             El[rm * Lh + i], Em2, w1x[0], w1y[0], w1z[0], c1J3, c1J4, w0z[1],
                             D0, D1);
       A1b[ring * P + slot] = F0;
-      XD1[ring * P + row * Lh + iw1] = make_ulonglong2(D0, D1);   // walked, read by B1 at k+2
+      if (!D2) XD1[ring * P + row * Lh + iw1] = make_ulonglong2(D0, D1);   // walked, read by B1 at k+2
     }
     if (rowA1 && k + 1 >= 2 && k + 1 < nstep) dA1 = ldo(Din1, h * 16u);   // next step's A1
 
     // ---- B0: second sweep, colour 0, plane s-2 (J of A0 two steps ago)
     if (rowB0 && k >= 4) {
-      const ulonglong2 d = XD0[ring2 * P + slot];
+      const ulonglong2 d = D2 ? dB0 : XD0[ring2 * P + slot];
       uint64_t D0 = d.x, D1 = d.y;
       const uint64_t *Fl = A1b + ring1 * P;
       const uint64_t Em2 = A0b[ring2 * P + slot];   // own spin: A0 at plane s-2
@@ -725,12 +804,13 @@ __global__ void __launch_bounds__(1024) k_sweep_pair(PairArgs a) {
       B0b[ring * P + slot] = G0;
       if (rowB1 && k >= 5 && k <= (int)a.tz + 4) {
         sto_stream(S0out, h2 * 8u, G0);
-        sto_stream(Dout0, (iw0 + yb + LhL * z2) * 16u, D0, D1);
+        if (a.dm_store) sto_stream(Dout0, (iw0 + yb + LhL * z2) * 16u, D0, D1);
       }
     }
+    if (D2 && rowB0 && k + 1 >= 4 && k + 1 < nstep) dB0 = ldo(Din20, (i + yb + LhL * z1) * 16u);
     // ---- B1: second sweep, colour 1, plane s-3 (the tile's own rows and planes)
     if (rowB1 && k >= 6) {
-      const ulonglong2 d = XD1[ring2 * P + slot];
+      const ulonglong2 d = D2 ? dB1 : XD1[ring2 * P + slot];
       uint64_t D0 = d.x, D1 = d.y;
       const uint64_t *Gl = B0b + ring1 * P;
       const uint64_t F2 = A1b[ring2 * P + slot];    // own spin: A1 at plane s-3
@@ -739,8 +819,9 @@ __global__ void __launch_bounds__(1024) k_sweep_pair(PairArgs a) {
                                            Gl[row * Lh + im], Gl[rm * Lh + i], G2, w1x[2], w1y[2],
                                            w1z[2], wc1x[1], wc1y[1], w0z[3], D0, D1);
       sto_stream(S1out, h3 * 8u, sn);
-      sto_stream(Dout1, (iw1 + yb + LhL * z3) * 16u, D0, D1);
+      if (a.dm_store) sto_stream(Dout1, (iw1 + yb + LhL * z3) * 16u, D0, D1);
     }
+    if (D2 && rowB1 && k + 1 >= 6 && k + 1 < nstep) dB1 = ldo(Din21, h2 * 16u);
     // ---- rotate the windows
     Sm = S0w;
     S0w = Sp;
@@ -1010,6 +1091,11 @@ static bool narrow_ok(const Geo &g) {
 // 16-byte demons.
 static size_t pair_lds(const Geo &g, uint32_t ty) { return (size_t)184 * (ty + 6) * g.Lh; }
 constexpr size_t kPairLdsMax = 160 * 1024;
+// The D2 form (second iteration refreshed): no demon rings, at most 512 threads.
+static size_t pair_lds_d2(const Geo &g, uint32_t ty) { return (size_t)88 * (ty + 6) * g.Lh; }
+static bool pair_d2_ok(const Geo &g, uint32_t ty) {
+  return (uint64_t)(ty + 6) * g.Lh <= 512 && pair_lds_d2(g, ty) <= kPairLdsMax;
+}
 
 // A sweep-pair tile is valid for one demon layer (its demons walk along x,
 // inside a block's rows), 32-bit offsets, ty and tz dividing L and a block
@@ -1103,7 +1189,11 @@ extern "C" int msa_create(int device, const msa_geom *geom, msa_ctx **out) {
                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)kRefreshLds) != hipSuccess ||
       hipFuncSetAttribute(reinterpret_cast<const void *>(&k_halfsweep<kRefresh, false>),
                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)kRefreshLds) != hipSuccess ||
-      hipFuncSetAttribute(reinterpret_cast<const void *>(&k_sweep_pair),
+      hipFuncSetAttribute(reinterpret_cast<const void *>(&k_refresh_draw),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)kDrawLds) != hipSuccess ||
+      hipFuncSetAttribute(reinterpret_cast<const void *>(&k_sweep_pair<false>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPairLdsMax) != hipSuccess ||
+      hipFuncSetAttribute(reinterpret_cast<const void *>(&k_sweep_pair<true>),
                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPairLdsMax) != hipSuccess)
     return cleanup(msa_fail(MSA_ERR_HIP, "kernel LDS attribute"));
   pick_half(c);
@@ -1117,8 +1207,15 @@ extern "C" int msa_destroy(msa_ctx *c) {
   if (!c) return MSA_OK;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->dstream) (void)hipStreamSynchronize(c->dstream);
   walkers_free(c);
   (void)hipFree(c->slab);
+  (void)hipFree(c->pre);
+  for (int h = 0; h < 2; ++h) {
+    if (c->e_drawn[h]) (void)hipEventDestroy(c->e_drawn[h]);
+    if (c->e_used[h]) (void)hipEventDestroy(c->e_used[h]);
+  }
+  if (c->dstream) (void)hipStreamDestroy(c->dstream);
   if (c->luts_owned) (void)hipFree(c->luts);
   (void)hipFree(c->ctr);
   (void)hipFree(c->key);
@@ -1193,6 +1290,22 @@ extern "C" int msa_get_tiling(msa_ctx *c, int32_t *mode, int32_t *ty, int32_t *t
   *mode = c->pair ? 3 : c->fused;
   *ty = (int32_t)(c->pair ? c->pty : c->fty);
   *tz = (int32_t)(c->pair ? c->ptz : c->ftz);
+  return MSA_OK;
+}
+
+extern "C" int msa_set_refresh_mode(msa_ctx *c, int32_t mode) {
+  if (!c) return msa_fail(MSA_ERR_ARG, "NULL context");
+  if (mode != 0 && mode != 1) return msa_fail(MSA_ERR_ARG, "refresh mode %d (0 or 1)", mode);
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (c->dstream) HIPCHK(hipStreamSynchronize(c->dstream));
+  c->predraw = mode;
+  return MSA_OK;
+}
+
+extern "C" int msa_get_refresh_mode(msa_ctx *c, int32_t *mode) {
+  if (!c || !mode) return msa_fail(MSA_ERR_ARG, "NULL argument");
+  *mode = c->predraw;
   return MSA_OK;
 }
 
@@ -1308,6 +1421,7 @@ extern "C" int msa_set_luts(msa_ctx *c, int32_t nlut, const msa_lut *luts) {
     if (!lut_valid(luts[k])) return msa_fail(MSA_ERR_ARG, "LUT %d is malformed", k);
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipStreamSynchronize(c->stream));
+  if (c->dstream) HIPCHK(hipStreamSynchronize(c->dstream));
   if (c->luts_owned) (void)hipFree(c->luts);
   c->luts = nullptr;
   c->luts_owned = true;
@@ -1335,6 +1449,7 @@ extern "C" int msa_set_refresh(msa_ctx *c, int32_t nref, const msa_u32x4 *ctr,
                   c->nlut - 1);
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipStreamSynchronize(c->stream));
+  if (c->dstream) HIPCHK(hipStreamSynchronize(c->dstream));
   c->nref = 0;
   const size_t R = c->g.R;
   if (nref > c->ctr_cap) {           // buffers are kept and reused when large enough
@@ -1378,27 +1493,177 @@ static void launch_fused(msa_ctx *c, const FusedArgs &fa) {
 
 // Two refresh-free iterations of a one-layer lattice in one pass.  The pass
 // reads the current buffers and writes the other ones, so each ping-pong
-// index flips once for the two iterations.
-static void launch_pair(msa_ctx *c) {
+// index flips once for the two iterations.  The first iteration's demons
+// come from dmi: the current buffer, or the demons drawn ahead for a refresh.
+// The second iteration takes the first one's walked demons, or, when it
+// refreshes (dmi2 not NULL, a tile pair_d2_ok accepts), the demons drawn for
+// its refresh.
+static void launch_pair(msa_ctx *c, const ulonglong2 *dmi, const ulonglong2 *dmi2, bool dm_store) {
   PairArgs pa{};
+  pa.dm_store = dm_store;
   pa.sp_in = c->sp[c->scur];
   pa.sp_out = c->sp[c->scur ^ 1];
   pa.jp = c->jp;
-  pa.dm_in = c->dm[c->cur];
+  pa.dm_in = dmi;
+  pa.dm_in2 = dmi2;
   pa.dm_out = c->dm[c->cur ^ 1];
   pa.g = c->g;
   pa.ty = c->pty;
   pa.tz = c->ptz;
   pa.ny = c->pty + 6;
   const dim3 grid((c->g.L / c->pty) * (c->g.L / c->ptz) * c->g.R), block(c->g.Lh * pa.ny);
-  hipLaunchKernelGGL(k_sweep_pair, grid, block, pair_lds(c->g, c->pty), c->stream, pa);
+  if (dmi2)
+    hipLaunchKernelGGL(k_sweep_pair<true>, grid, block, pair_lds_d2(c->g, c->pty), c->stream, pa);
+  else
+    hipLaunchKernelGGL(k_sweep_pair<false>, grid, block, pair_lds(c->g, c->pty), c->stream, pa);
   c->scur ^= 1;
   c->cur ^= 1;
+}
+
+// Refreshes before iteration t (the index of the first refresh at or after t).
+static int64_t refreshes_before(uint64_t t, int64_t every) {
+  if (every > 0) return (int64_t)((t + (uint64_t)every - 1) / (uint64_t)every);
+  if (every == -1) {
+    if (t == 0) return 0;
+    int64_t k = 1;                        // refresh k >= 1 is at iteration 2^(k-1)
+    while (k < 64 && ((uint64_t)1 << (k - 1)) < t) ++k;
+    return k;
+  }
+  return 0;
+}
+
+// Demons drawn ahead (predraw mode): words of one refresh, R nc V.
+static size_t pre_words(const msa_ctx *c) { return (size_t)c->g.R * c->g.nc * c->g.V; }
+
+// Stream, events and the two halves of the drawn-demon buffer, made on the
+// first run in predraw mode: each half holds pK refreshes, at most 256 MiB
+// (at least one refresh) and kDrawMax.
+static int predraw_setup(msa_ctx *c) {
+  if (c->pre) return MSA_OK;
+  const size_t per = pre_words(c) * sizeof(ulonglong2);
+  size_t K = ((size_t)256 << 20) / per;
+  if (K < 1) K = 1;
+  if (K > (size_t)kDrawMax) K = kDrawMax;
+  if (!c->dstream && hipStreamCreateWithFlags(&c->dstream, hipStreamNonBlocking) != hipSuccess)
+    return msa_fail(MSA_ERR_HIP, "draw stream creation failed");
+  for (int h = 0; h < 2; ++h)
+    if ((!c->e_drawn[h] && hipEventCreateWithFlags(&c->e_drawn[h], hipEventDisableTiming) != hipSuccess) ||
+        (!c->e_used[h] && hipEventCreateWithFlags(&c->e_used[h], hipEventDisableTiming) != hipSuccess))
+      return msa_fail(MSA_ERR_HIP, "draw event creation failed");
+  int ncu = 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || ncu < 1)
+    ncu = 256;
+  c->ncu = (uint32_t)ncu;
+  if (hipMalloc(&c->pre, 2 * K * per) != hipSuccess) {
+    c->pre = nullptr;
+    return msa_fail(MSA_ERR_NOMEM, "drawn-demon buffer of %lld bytes", (long long)(2 * K * per));
+  }
+  c->pK = (uint32_t)K;
+  return MSA_OK;
+}
+
+// The refreshes [kb, ke) of one msa_run call, drawn pK at a time into the
+// two halves of c->pre (chunk j: refreshes kb + j pK ..., half j & 1).  The
+// sweeps that read chunk j wait for its draw (e_drawn[j & 1]); once every
+// kernel reading chunk j is queued, e_used[j & 1] is recorded after them, and
+// the draw of chunk j + 2 into the same half waits for it.  Draws run one
+// chunk ahead of the sweeps, so a draw overlaps the sweeps of the chunk
+// before it.
+struct Predraw {
+  int64_t kb = 0, ke = 0;
+  int64_t drawn = 0;        // chunks whose draw is queued
+  int64_t waited = -1;      // last chunk the sweep stream waits for
+  int64_t released = 0;     // chunks whose readers are all queued (e_used recorded)
+  int64_t chunks = 0;       // chunks of the call
+};
+
+static int draw_chunk(msa_ctx *c, const Predraw &pd, int64_t jj) {
+  const int64_t k0 = pd.kb + jj * (int64_t)c->pK;
+  const int64_t nk = pd.ke - k0 < (int64_t)c->pK ? pd.ke - k0 : (int64_t)c->pK;
+  HIPCHK(hipStreamWaitEvent(c->dstream, c->e_used[jj & 1], 0));
+  DrawArgs a{};
+  a.out = c->pre + (size_t)(jj & 1) * c->pK * pre_words(c);
+  a.ctr = c->ctr + (size_t)k0 * c->g.R;
+  a.key = c->key;
+  a.luts = c->luts;
+  a.g = c->g;
+  const uint32_t Lh = c->g.Lh, rows = c->g.L * c->g.L;
+  a.nr = 768 / Lh < 1 ? 1 : 768 / Lh;
+  if (a.nr > rows) a.nr = rows;
+  a.mLh = div_recip_of(Lh);
+  a.mL = div_recip_of(c->g.L);
+  for (int64_t j = 0; j < nk; ++j) a.lidx[j] = c->lut_idx[(size_t)(k0 + j)];
+  // one block per CU: blocks of at most 768 threads (three waves per SIMD,
+  // 80 VGPRs each) with more than half of the CU's LDS, so a sweep block of
+  // up to two waves per SIMD at 136 VGPRs still fits beside them; the CUs
+  // split over the launch's refreshes
+  const uint32_t nbr = (rows + a.nr - 1) / a.nr;
+  uint32_t gx = (c->ncu + (uint32_t)nk - 1) / (uint32_t)nk;
+  if (gx > nbr) gx = nbr;
+  if (gx < 1) gx = 1;
+  hipLaunchKernelGGL(k_refresh_draw, dim3(gx, (uint32_t)nk), dim3(a.nr * Lh), kDrawLds,
+                     c->dstream, a);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(c->e_drawn[jj & 1], c->dstream));
+  return MSA_OK;
+}
+
+// Queue the draws up to chunk `upto` (at most the call's last chunk) whose
+// half is free: chunk jj needs chunk jj - 2 released.
+static int draw_ahead(msa_ctx *c, Predraw &pd, int64_t upto) {
+  if (upto >= pd.chunks) upto = pd.chunks - 1;
+  while (pd.drawn <= upto && pd.drawn < pd.released + 2) {
+    const int rc = draw_chunk(c, pd, pd.drawn);
+    if (rc) return rc;
+    ++pd.drawn;
+  }
+  return MSA_OK;
+}
+
+// The drawn demons of refresh k, the sweep stream made to wait for them.
+static int drawn_demons(msa_ctx *c, Predraw &pd, int64_t k, const ulonglong2 **out) {
+  const int64_t j = (k - pd.kb) / (int64_t)c->pK;
+  int rc = draw_ahead(c, pd, j + 1);
+  if (rc) return rc;
+  if (pd.drawn <= j) return msa_fail(MSA_ERR_STATE, "refresh %lld: its draw is not queued", (long long)k);
+  if (j > pd.waited) {
+    HIPCHK(hipStreamWaitEvent(c->stream, c->e_drawn[j & 1], 0));
+    pd.waited = j;
+  }
+  *out = c->pre + ((size_t)(j & 1) * c->pK + (size_t)(k - pd.kb - j * (int64_t)c->pK)) * pre_words(c);
+  return MSA_OK;
+}
+
+// Every kernel reading refreshes below k_next is queued: release their
+// chunks and draw ahead into the halves that frees.
+static int release_before(msa_ctx *c, Predraw &pd, int64_t k_next) {
+  const int64_t j = (k_next - pd.kb) / (int64_t)c->pK;
+  while (pd.released < j && pd.released < pd.drawn) {
+    HIPCHK(hipEventRecord(c->e_used[pd.released & 1], c->stream));
+    ++pd.released;
+  }
+  return draw_ahead(c, pd, j + 1);
 }
 
 static int launch_run(msa_ctx *c, uint64_t t0, int64_t niter, int64_t every) {
   if (niter < 0) return msa_fail(MSA_ERR_ARG, "niter=%lld", (long long)niter);
   if (every < -1) return msa_fail(MSA_ERR_ARG, "refresh_every=%lld", (long long)every);
+  Predraw pd;
+  const bool pdraw = c->predraw && c->w_n == 0 && c->nlut > 0;
+  if (pdraw) {
+    pd.kb = refreshes_before(t0, every);
+    pd.ke = refreshes_before(t0 + (uint64_t)niter, every);
+    if (pd.ke > c->nref) pd.ke = c->nref;  // the loop reports the missing refresh
+    if (pd.ke > pd.kb) {
+      int rc = predraw_setup(c);
+      if (rc) return rc;
+      pd.chunks = (pd.ke - pd.kb + c->pK - 1) / c->pK;
+      // both halves are free once the work queued so far has run
+      HIPCHK(hipEventRecord(c->e_used[0], c->stream));
+      HIPCHK(hipEventRecord(c->e_used[1], c->stream));
+    }
+  }
+  const bool d2 = pdraw && c->pair && pair_d2_ok(c->g, c->pty);
   for (int64_t n = 0; n < niter; ++n) {
     const uint64_t t = t0 + (uint64_t)n;
     int64_t k = refresh_index(t, every);
@@ -1410,17 +1675,42 @@ static int launch_run(msa_ctx *c, uint64_t t0, int64_t niter, int64_t every) {
     if (k >= 0 && (k >= c->nref || c->nlut == 0))
       return msa_fail(MSA_ERR_RANGE, "iteration %llu needs refresh %lld but %d are set",
                   (unsigned long long)t, (long long)k, c->nref);
-    if (k < 0 && c->pair && n + 1 < niter && refresh_index(t + 1, every) < 0) {
-      launch_pair(c);
-      ++n;
-      continue;
-    }
     const ulonglong2 *dmi = c->dm[c->cur];
+    const bool drawn = k >= 0 && pdraw;
+    if (drawn) {
+      // the refresh's demons, drawn ahead, are this sweep's input demons
+      const int rc = drawn_demons(c, pd, k, &dmi);
+      if (rc) return rc;
+    }
+    if ((k < 0 || drawn) && c->pair && n + 1 < niter) {
+      const int64_t k2 = refresh_index(t + 1, every);
+      const bool next_drawn = k2 >= 0 && d2 && k2 < pd.ke;
+      if (k2 < 0 || next_drawn) {
+        const ulonglong2 *dmi2 = nullptr;
+        if (next_drawn) {
+          const int rc = drawn_demons(c, pd, k2, &dmi2);
+          if (rc) return rc;
+        }
+        // walked demons the next iteration's drawn refresh replaces are not written
+        const int64_t k3 = n + 2 < niter ? refresh_index(t + 2, every) : -1;
+        launch_pair(c, dmi, dmi2, !(pdraw && k3 >= 0 && k3 < pd.ke));
+        if (drawn || next_drawn) {
+          const int rc = release_before(c, pd, (next_drawn ? k2 : k) + 1);
+          if (rc) return rc;
+        }
+        ++n;
+        continue;
+      }
+    }
+    // a refresh drawn ahead runs the refresh-free kernels
+    const int64_t kf = drawn ? -1 : k;
+    const int64_t kn = n + 1 < niter ? refresh_index(t + 1, every) : -1;
+    const bool dm_store = !(pdraw && kn >= 0 && kn < pd.ke);
     ulonglong2 *dmo = c->dm[c->cur ^ 1];
     for (uint32_t l = 0; l < c->g.nc; ++l) {
       const uint64_t *A = c->sp[c->scur];
       uint64_t *B = c->sp[c->scur ^ 1];
-      if (k < 0 && c->fused) {
+      if (kf < 0 && c->fused) {
         FusedArgs fa{};
         fa.sp_in = A;
         fa.sp_out = B;
@@ -1433,6 +1723,7 @@ static int launch_run(msa_ctx *c, uint64_t t0, int64_t niter, int64_t every) {
         fa.tz = c->ftz;
         fa.ny = c->fny;
         fa.yhalo = c->fyhalo;
+        fa.dm_store = dm_store;
         launch_fused(c, fa);
       } else {
         SweepArgs a{};
@@ -1446,10 +1737,10 @@ static int launch_run(msa_ctx *c, uint64_t t0, int64_t niter, int64_t every) {
         a.mLh = div_recip_of(c->g.Lh);
         a.mL = div_recip_of(c->g.L);
         a.mG = div_recip_of(c->hnr * c->g.Lh);
-        if (k >= 0) {
-          a.ctr = c->ctr + (size_t)k * c->g.R;
+        if (kf >= 0) {
+          a.ctr = c->ctr + (size_t)kf * c->g.R;
           a.key = c->key;
-          a.lut = c->luts + 2 * (size_t)c->lut_idx[k];
+          a.lut = c->luts + 2 * (size_t)c->lut_idx[kf];
         }
         const dim3 grid(c->hgrid), block(c->hnr * c->g.Lh * c->hrs);
         for (int col = 0; col < 2; ++col) {
@@ -1458,9 +1749,9 @@ static int launch_run(msa_ctx *c, uint64_t t0, int64_t niter, int64_t every) {
           a.sp_nb = col ? B : A;
           a.sp_out = B;
           const bool one = c->hrs == c->g.R;   // also R = 1
-          if (k >= 0 && one)
+          if (kf >= 0 && one)
             hipLaunchKernelGGL((k_halfsweep<kRefresh, true>), grid, block, kRefreshLds, c->stream, a);
-          else if (k >= 0)
+          else if (kf >= 0)
             hipLaunchKernelGGL((k_halfsweep<kRefresh, false>), grid, block, kRefreshLds, c->stream, a);
           else if (one)
             hipLaunchKernelGGL((k_halfsweep<kMicro, true>), grid, block, 0, c->stream, a);
@@ -1471,6 +1762,10 @@ static int launch_run(msa_ctx *c, uint64_t t0, int64_t niter, int64_t every) {
       c->scur ^= 1;
     }
     c->cur ^= 1;
+    if (drawn) {
+      const int rc = release_before(c, pd, k + 1);
+      if (rc) return rc;
+    }
   }
   HIPCHK(hipGetLastError());
   return MSA_OK;

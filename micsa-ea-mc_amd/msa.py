@@ -70,6 +70,8 @@ def hip():
         L.msa_sync.argtypes = [vp]
         L.msa_set_tiling.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32]
         L.msa_get_tiling.argtypes = [vp, i32p, i32p, i32p]
+        L.msa_set_refresh_mode.argtypes = [vp, C.c_int32]
+        L.msa_get_refresh_mode.argtypes = [vp, i32p]
         L.msa_run_thermal.argtypes = [vp, C.c_int64, C.c_void_p, u32p, u32p]
         L.msa_run_thermal_noalloc.argtypes = [vp, C.c_int64, C.c_void_p, u32p, u32p]
         L.msa_set_walker_streams.argtypes = [vp, C.c_int32, i32p, C.c_void_p, C.c_int32, u64p]
@@ -225,6 +227,15 @@ class Lattice:
         _check(self.lib.msa_autotune(self.ctx, sweeps, C.byref(m), C.byref(ty), C.byref(tz),
                                      C.byref(ms)))
         return m.value, ty.value, tz.value, ms.value
+
+    def set_refresh_mode(self, mode):
+        """0: refresh drawn inside the sweep; 1: drawn ahead on a second stream."""
+        _check(self.lib.msa_set_refresh_mode(self.ctx, mode))
+
+    def refresh_mode(self):
+        m = C.c_int32()
+        _check(self.lib.msa_get_refresh_mode(self.ctx, C.byref(m)))
+        return m.value
 
     def tiling(self):
         m, ty, tz = C.c_int32(), C.c_int32(), C.c_int32()
