@@ -423,8 +423,10 @@ def main():
                     help="also time a streaming copy of this many MiB (bandwidth calibration)")
     ap.add_argument("--tiling", type=str, default="",
                     help="MODE,TY,TZ: use this sweep kernel and tile (no autotune)")
-    ap.add_argument("--refresh-mode", type=int, default=0,
-                    help="refresh sweeps: 0 drawn inside the sweep, 1 drawn ahead on a second stream")
+    ap.add_argument("--refresh-mode", type=int, default=-1,
+                    help="refresh sweeps: 0 drawn inside the sweep, 1 drawn ahead on a second "
+                         "stream, 2 as 1 with two refresh sweeps per pass, -1 the library's "
+                         "choice for the geometry (after the autotune)")
     ap.add_argument("--thermal-iters", type=int, default=0,
                     help="also time this many Metropolis thermalization iterations (msa_run_thermal)")
     args = ap.parse_args()
@@ -546,6 +548,7 @@ def main():
         walkers = {"refreshes": args.walker_refreshes, "samples": 2,
                    "ms_per_refresh": round(float(np.mean(both)) - plain, 4),
                    "ms_sweep_alone": round(plain, 4)}
+    rmode = lat.refresh_mode()
     lat.close()
     copy = None
     if args.copy_probe_mb > 0:
@@ -590,6 +593,8 @@ def main():
             "kernel_mode": ["two-pass", "one-pass", "one-pass, single-load",
                             "two sweeps per pass"][tiling[0]],
             "tile_rows_planes": [tiling[1], tiling[2]],
+            "refresh_mode": ["drawn inside the sweep", "drawn ahead on a second stream",
+                             "drawn ahead, two refresh sweeps per pass"][rmode],
         },
         "roofline": {
             "bound": "hbm",

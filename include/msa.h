@@ -194,11 +194,17 @@ int msa_get_tiling(msa_ctx *ctx, int32_t *mode, int32_t *ty, int32_t *tz);
  * mode 0: the refresh is drawn inside the sweep's demon load (two kernels per
  * sweep, one per colour).  mode 1: the demons of the coming refreshes are
  * drawn ahead, up to 64 refreshes per launch, by a separate kernel on a
- * second stream of the context (it overlaps the sweeps before them; at most
- * 512 MiB of device memory, allocated on the first such run), and each
- * refresh sweep runs the refresh-free kernels of msa_set_tiling reading them.
- * Results are identical in both modes.  Walker-stream refreshes are not
- * affected. */
+ * second stream of the context (it overlaps the sweeps before them, also
+ * across msa_run calls; at most 512 MiB of device memory, allocated on the
+ * first such run), and each refresh sweep runs the refresh-free kernels
+ * reading them; a refresh sweep followed by a refresh-free one runs as the
+ * msa_set_tiling sweep pair.  mode 2: as mode 1, and two refresh sweeps in a
+ * row run as one sweep pair of at most 512 threads per block (nc = 1, L up to
+ * 144; elsewhere as mode 1).  mode -1 (the default): the mode msa_autotune
+ * measured fastest for the geometry, or before any autotune 2 where the
+ * pair's tile has at least 6 rows, 1 where it exists or L <= 48, else 0.
+ * Results are identical in every mode; msa_get_refresh_mode gives the mode in
+ * effect (0, 1 or 2).  Walker-stream refreshes are not affected. */
 int msa_set_refresh_mode(msa_ctx *ctx, int32_t mode);
 int msa_get_refresh_mode(msa_ctx *ctx, int32_t *mode);
 
@@ -206,7 +212,9 @@ int msa_get_refresh_mode(msa_ctx *ctx, int32_t *mode);
  * kernel choice valid for this geometry on a scratch lattice of the same
  * geometry (the context's own state is not touched), applies the fastest and
  * remembers it for the process; later calls for the same (L, R, nc) reuse it.
- * Outputs may be NULL. */
+ * It then times sweeps with a refresh each in refresh modes 0, 1 and 2 with
+ * that kernel choice and makes the fastest the context's default refresh
+ * mode (msa_set_refresh_mode(-1)).  Outputs may be NULL. */
 int msa_autotune(msa_ctx *ctx, int32_t sweeps, int32_t *mode, int32_t *ty, int32_t *tz,
                  float *ms_per_sweep);
 

@@ -5,6 +5,15 @@
 
 #include "msa_common.hpp"
 
+// The drawn-ahead refresh pipeline of a context (msa_hip.hip, launch_run).
+struct Predraw {
+  int64_t kb = 0;           // refresh index of chunk 0
+  int64_t drawn = 0;        // chunks whose draw is queued
+  int64_t waited = -1;      // last chunk the sweep stream waits for
+  int64_t released = 0;     // chunks whose readers are all queued (e_used recorded)
+  bool live = false;        // the drawn chunks match the current tables
+};
+
 struct msa_ctx {
   int device = 0;
   msa_geom geom{};
@@ -36,13 +45,17 @@ struct msa_ctx {
   uint32_t pty = 0, ptz = 0;              // its tile: rows, planes
   // Philox refresh sweeps: 0 = the refresh fused into the two-pass kernel's
   // demon load; 1 = demons drawn ahead by k_refresh_draw on a second stream
-  // and read by the refresh-free kernels
-  int predraw = 0;
+  // and read by the refresh-free kernels; 2 = as 1, two refresh sweeps in a
+  // row as one D2 sweep pair; -1 = chosen by geometry
+  int predraw = -1;
+  uint32_t qty = 0, qtz = 0;              // tile of the D2 sweep pair (0: none)
+  int tuned_rmode = -1;                   // refresh mode msa_autotune chose (-1: none)
   hipStream_t dstream = nullptr;
   hipEvent_t e_drawn[2] = {nullptr, nullptr}, e_used[2] = {nullptr, nullptr};
   ulonglong2 *pre = nullptr;              // [2 pK][R nc V] drawn demons (two halves)
   uint32_t pK = 0;                        // refreshes per half
   uint32_t ncu = 0;                       // compute units of the device
+  Predraw pd;
   // demon refresh from the reference's walker streams (msa_walkers.hip)
   int32_t w_n = 0;                        // streams per replica, 0 = Philox refresh
   int w_cur = 0;                          // w_st buffer holding the current states

@@ -1096,6 +1096,8 @@ static size_t pair_lds_d2(const Geo &g, uint32_t ty) { return (size_t)88 * (ty +
 static bool pair_d2_ok(const Geo &g, uint32_t ty) {
   return (uint64_t)(ty + 6) * g.Lh <= 512 && pair_lds_d2(g, ty) <= kPairLdsMax;
 }
+static void pick_pair_d2(const Geo &g, uint32_t &bty, uint32_t &btz);
+static int refresh_mode(const msa_ctx *c);
 
 // A sweep-pair tile is valid for one demon layer (its demons walk along x,
 // inside a block's rows), 32-bit offsets, ty and tz dividing L and a block
@@ -1198,6 +1200,7 @@ extern "C" int msa_create(int device, const msa_geom *geom, msa_ctx **out) {
     return cleanup(msa_fail(MSA_ERR_HIP, "kernel LDS attribute"));
   pick_half(c);
   pick_fused(c);
+  pick_pair_d2(c->g, c->qty, c->qtz);
   if ((rc = thermal_scratch_init(device))) return cleanup(rc);
   *out = c;
   return MSA_OK;
@@ -1246,6 +1249,25 @@ static int pick_pair(const Geo &g, uint32_t &bty, uint32_t &btz) {
   return bty != 0;
 }
 
+// Tile of the D2 sweep pair (a pair whose second iteration refreshes, with
+// demons drawn ahead): the pick_pair score over the tiles of at most 512
+// threads, ties to the taller tile; ty = 0 when the geometry has none.
+static void pick_pair_d2(const Geo &g, uint32_t &bty, uint32_t &btz) {
+  double best = -1;
+  bty = btz = 0;
+  for (uint32_t ty = g.L; ty >= 1; --ty)
+    for (uint32_t tz = 4; tz <= g.L && tz <= 64; ++tz) {
+      if (!pair_tile_ok(g, ty, tz) || !pair_d2_ok(g, ty)) continue;
+      const double blocks = (double)(g.L / ty) * (g.L / tz) * g.R;
+      double work = 0;
+      for (int p = 0; p < 4; ++p) work += (ty + 6.0 - 2 * p) * (tz + 6.0 - 2 * p);
+      const double waste = work / (4.0 * ty * tz);
+      const double fill = blocks >= 256 ? 1.0 : blocks / 256.0;
+      const double score = fill / waste;
+      if (score > best) { best = score; bty = ty; btz = tz; }
+    }
+}
+
 extern "C" int msa_set_tiling(msa_ctx *c, int32_t mode, int32_t ty, int32_t tz) {
   if (!c) return msa_fail(MSA_ERR_ARG, "NULL context");
   if (mode == 3) {
@@ -1256,8 +1278,8 @@ extern "C" int msa_set_tiling(msa_ctx *c, int32_t mode, int32_t ty, int32_t tz) 
       return msa_fail(MSA_ERR_ARG, "sweep-pair tile ty=%d tz=%d invalid for L=%u R=%u nc=%u", ty, tz,
                       c->g.L, c->g.R, c->g.nc);
     // a sweep left over between refreshes runs on the one-pass kernel where
-    // it exists for this R, else on the two-pass form
-    if (c->fused == 0) pick_fused(c);
+    // it exists for this R (its default tile), else on the two-pass form
+    pick_fused(c);
     c->pair = 1;
     c->pty = y;
     c->ptz = z;
@@ -1295,17 +1317,18 @@ extern "C" int msa_get_tiling(msa_ctx *c, int32_t *mode, int32_t *ty, int32_t *t
 
 extern "C" int msa_set_refresh_mode(msa_ctx *c, int32_t mode) {
   if (!c) return msa_fail(MSA_ERR_ARG, "NULL context");
-  if (mode != 0 && mode != 1) return msa_fail(MSA_ERR_ARG, "refresh mode %d (0 or 1)", mode);
+  if (mode < -1 || mode > 2) return msa_fail(MSA_ERR_ARG, "refresh mode %d (-1 .. 2)", mode);
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipStreamSynchronize(c->stream));
   if (c->dstream) HIPCHK(hipStreamSynchronize(c->dstream));
+  c->pd.live = false;
   c->predraw = mode;
   return MSA_OK;
 }
 
 extern "C" int msa_get_refresh_mode(msa_ctx *c, int32_t *mode) {
   if (!c || !mode) return msa_fail(MSA_ERR_ARG, "NULL argument");
-  *mode = c->predraw;
+  *mode = refresh_mode(c);
   return MSA_OK;
 }
 
@@ -1422,6 +1445,7 @@ extern "C" int msa_set_luts(msa_ctx *c, int32_t nlut, const msa_lut *luts) {
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipStreamSynchronize(c->stream));
   if (c->dstream) HIPCHK(hipStreamSynchronize(c->dstream));
+  c->pd.live = false;
   if (c->luts_owned) (void)hipFree(c->luts);
   c->luts = nullptr;
   c->luts_owned = true;
@@ -1433,6 +1457,8 @@ extern "C" int msa_set_luts(msa_ctx *c, int32_t nlut, const msa_lut *luts) {
 }
 
 int msa_ctx_share_luts(msa_ctx *c, msa::FineLut *luts, int32_t nlut) {
+  if (c->dstream) (void)hipStreamSynchronize(c->dstream);
+  c->pd.live = false;
   if (c->luts_owned) (void)hipFree(c->luts);
   c->luts = luts;
   c->nlut = nlut;
@@ -1450,6 +1476,7 @@ extern "C" int msa_set_refresh(msa_ctx *c, int32_t nref, const msa_u32x4 *ctr,
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipStreamSynchronize(c->stream));
   if (c->dstream) HIPCHK(hipStreamSynchronize(c->dstream));
+  c->pd.live = false;
   c->nref = 0;
   const size_t R = c->g.R;
   if (nref > c->ctr_cap) {           // buffers are kept and reused when large enough
@@ -1508,16 +1535,31 @@ static void launch_pair(msa_ctx *c, const ulonglong2 *dmi, const ulonglong2 *dmi
   pa.dm_in2 = dmi2;
   pa.dm_out = c->dm[c->cur ^ 1];
   pa.g = c->g;
-  pa.ty = c->pty;
-  pa.tz = c->ptz;
-  pa.ny = c->pty + 6;
-  const dim3 grid((c->g.L / c->pty) * (c->g.L / c->ptz) * c->g.R), block(c->g.Lh * pa.ny);
+  pa.ty = dmi2 ? c->qty : c->pty;
+  pa.tz = dmi2 ? c->qtz : c->ptz;
+  pa.ny = pa.ty + 6;
+  const dim3 grid((c->g.L / pa.ty) * (c->g.L / pa.tz) * c->g.R), block(c->g.Lh * pa.ny);
   if (dmi2)
-    hipLaunchKernelGGL(k_sweep_pair<true>, grid, block, pair_lds_d2(c->g, c->pty), c->stream, pa);
+    hipLaunchKernelGGL(k_sweep_pair<true>, grid, block, pair_lds_d2(c->g, pa.ty), c->stream, pa);
   else
     hipLaunchKernelGGL(k_sweep_pair<false>, grid, block, pair_lds(c->g, c->pty), c->stream, pa);
   c->scur ^= 1;
   c->cur ^= 1;
+}
+
+// The refresh mode msa_run uses (0 fused, 1 drawn ahead, 2 drawn ahead with
+// D2 pairs): the one set, or for -1 (the default) the one msa_autotune
+// measured fastest for this geometry, else demons drawn ahead where that
+// measured faster -- geometries with a D2 sweep-pair tile (L up to 144) and
+// small lattices (L <= 48), whose fused refresh sweeps are bound by their
+// launches.
+static int refresh_mode(const msa_ctx *c) {
+  if (c->predraw >= 0) return c->predraw;
+  if (c->tuned_rmode >= 0) return c->tuned_rmode;
+  // untuned: D2 pairs where their tile has at least 6 own rows (thinner tiles
+  // recompute so much halo that single one-pass sweeps measured faster)
+  if (c->qty >= 6) return 2;
+  return (c->qty > 0 || c->g.L <= 48) ? 1 : 0;
 }
 
 // Refreshes before iteration t (the index of the first refresh at or after t).
@@ -1562,24 +1604,18 @@ static int predraw_setup(msa_ctx *c) {
   return MSA_OK;
 }
 
-// The refreshes [kb, ke) of one msa_run call, drawn pK at a time into the
-// two halves of c->pre (chunk j: refreshes kb + j pK ..., half j & 1).  The
-// sweeps that read chunk j wait for its draw (e_drawn[j & 1]); once every
-// kernel reading chunk j is queued, e_used[j & 1] is recorded after them, and
-// the draw of chunk j + 2 into the same half waits for it.  Draws run one
-// chunk ahead of the sweeps, so a draw overlaps the sweeps of the chunk
-// before it.
-struct Predraw {
-  int64_t kb = 0, ke = 0;
-  int64_t drawn = 0;        // chunks whose draw is queued
-  int64_t waited = -1;      // last chunk the sweep stream waits for
-  int64_t released = 0;     // chunks whose readers are all queued (e_used recorded)
-  int64_t chunks = 0;       // chunks of the call
-};
-
+// Refreshes kb, kb + 1, ... drawn pK at a time into the two halves of c->pre
+// (chunk j: refreshes kb + j pK ..., half j & 1), as far as the refresh
+// tables reach.  The sweeps that read chunk j wait for its draw
+// (e_drawn[j & 1]); once every kernel reading chunk j is queued, e_used[j & 1]
+// is recorded after them, and the draw of chunk j + 2 into the same half
+// waits for it.  Draws run one chunk ahead of the sweeps, so a draw overlaps
+// the sweeps of the chunk before it, also across msa_run calls: a call whose
+// first refresh lies in a chunk already drawn (or the next one) continues the
+// pipeline (c->pd); new tables or a new mode end it.
 static int draw_chunk(msa_ctx *c, const Predraw &pd, int64_t jj) {
   const int64_t k0 = pd.kb + jj * (int64_t)c->pK;
-  const int64_t nk = pd.ke - k0 < (int64_t)c->pK ? pd.ke - k0 : (int64_t)c->pK;
+  const int64_t nk = c->nref - k0 < (int64_t)c->pK ? c->nref - k0 : (int64_t)c->pK;
   HIPCHK(hipStreamWaitEvent(c->dstream, c->e_used[jj & 1], 0));
   DrawArgs a{};
   a.out = c->pre + (size_t)(jj & 1) * c->pK * pre_words(c);
@@ -1608,10 +1644,11 @@ static int draw_chunk(msa_ctx *c, const Predraw &pd, int64_t jj) {
   return MSA_OK;
 }
 
-// Queue the draws up to chunk `upto` (at most the call's last chunk) whose
-// half is free: chunk jj needs chunk jj - 2 released.
+// Queue the draws up to chunk `upto` (at most the last chunk the tables
+// reach) whose half is free: chunk jj needs chunk jj - 2 released.
 static int draw_ahead(msa_ctx *c, Predraw &pd, int64_t upto) {
-  if (upto >= pd.chunks) upto = pd.chunks - 1;
+  const int64_t chunks = (c->nref - pd.kb + c->pK - 1) / (int64_t)c->pK;
+  if (upto >= chunks) upto = chunks - 1;
   while (pd.drawn <= upto && pd.drawn < pd.released + 2) {
     const int rc = draw_chunk(c, pd, pd.drawn);
     if (rc) return rc;
@@ -1648,22 +1685,33 @@ static int release_before(msa_ctx *c, Predraw &pd, int64_t k_next) {
 static int launch_run(msa_ctx *c, uint64_t t0, int64_t niter, int64_t every) {
   if (niter < 0) return msa_fail(MSA_ERR_ARG, "niter=%lld", (long long)niter);
   if (every < -1) return msa_fail(MSA_ERR_ARG, "refresh_every=%lld", (long long)every);
-  Predraw pd;
-  const bool pdraw = c->predraw && c->w_n == 0 && c->nlut > 0;
+  const int rmode = refresh_mode(c);
+  const bool pdraw = rmode >= 1 && c->w_n == 0 && c->nlut > 0;
+  Predraw &pd = c->pd;
+  int64_t ke = 0;                          // refreshes below ke are this call's
   if (pdraw) {
-    pd.kb = refreshes_before(t0, every);
-    pd.ke = refreshes_before(t0 + (uint64_t)niter, every);
-    if (pd.ke > c->nref) pd.ke = c->nref;  // the loop reports the missing refresh
-    if (pd.ke > pd.kb) {
+    const int64_t kb = refreshes_before(t0, every);
+    ke = refreshes_before(t0 + (uint64_t)niter, every);
+    if (ke > c->nref) ke = c->nref;        // the loop reports the missing refresh
+    if (ke > kb) {
       int rc = predraw_setup(c);
       if (rc) return rc;
-      pd.chunks = (pd.ke - pd.kb + c->pK - 1) / c->pK;
-      // both halves are free once the work queued so far has run
-      HIPCHK(hipEventRecord(c->e_used[0], c->stream));
-      HIPCHK(hipEventRecord(c->e_used[1], c->stream));
+      const int64_t j0 = pd.live && kb >= pd.kb ? (kb - pd.kb) / (int64_t)c->pK : -1;
+      if (j0 >= pd.released && j0 <= pd.drawn) {
+        // the pipeline goes on: chunks before this call's first are done
+        rc = release_before(c, pd, kb);
+        if (rc) return rc;
+      } else {
+        pd = Predraw{};
+        pd.kb = kb;
+        pd.live = true;
+        // both halves are free once the work queued so far has run
+        HIPCHK(hipEventRecord(c->e_used[0], c->stream));
+        HIPCHK(hipEventRecord(c->e_used[1], c->stream));
+      }
     }
   }
-  const bool d2 = pdraw && c->pair && pair_d2_ok(c->g, c->pty);
+  const bool d2 = pdraw && rmode == 2 && c->qty > 0;
   for (int64_t n = 0; n < niter; ++n) {
     const uint64_t t = t0 + (uint64_t)n;
     int64_t k = refresh_index(t, every);
@@ -1682,10 +1730,10 @@ static int launch_run(msa_ctx *c, uint64_t t0, int64_t niter, int64_t every) {
       const int rc = drawn_demons(c, pd, k, &dmi);
       if (rc) return rc;
     }
-    if ((k < 0 || drawn) && c->pair && n + 1 < niter) {
+    if ((k < 0 || drawn) && n + 1 < niter) {
       const int64_t k2 = refresh_index(t + 1, every);
-      const bool next_drawn = k2 >= 0 && d2 && k2 < pd.ke;
-      if (k2 < 0 || next_drawn) {
+      const bool next_drawn = k2 >= 0 && d2 && k2 < ke;
+      if ((k2 < 0 && c->pair) || next_drawn) {
         const ulonglong2 *dmi2 = nullptr;
         if (next_drawn) {
           const int rc = drawn_demons(c, pd, k2, &dmi2);
@@ -1693,7 +1741,7 @@ static int launch_run(msa_ctx *c, uint64_t t0, int64_t niter, int64_t every) {
         }
         // walked demons the next iteration's drawn refresh replaces are not written
         const int64_t k3 = n + 2 < niter ? refresh_index(t + 2, every) : -1;
-        launch_pair(c, dmi, dmi2, !(pdraw && k3 >= 0 && k3 < pd.ke));
+        launch_pair(c, dmi, dmi2, !(pdraw && k3 >= 0 && k3 < ke));
         if (drawn || next_drawn) {
           const int rc = release_before(c, pd, (next_drawn ? k2 : k) + 1);
           if (rc) return rc;
@@ -1705,7 +1753,7 @@ static int launch_run(msa_ctx *c, uint64_t t0, int64_t niter, int64_t every) {
     // a refresh drawn ahead runs the refresh-free kernels
     const int64_t kf = drawn ? -1 : k;
     const int64_t kn = n + 1 < niter ? refresh_index(t + 1, every) : -1;
-    const bool dm_store = !(pdraw && kn >= 0 && kn < pd.ke);
+    const bool dm_store = !(pdraw && kn >= 0 && kn < ke);
     ulonglong2 *dmo = c->dm[c->cur ^ 1];
     for (uint32_t l = 0; l < c->g.nc; ++l) {
       const uint64_t *A = c->sp[c->scur];
@@ -2053,8 +2101,60 @@ struct TuneEntry {
   uint32_t L, R, nc;
   int32_t mode, ty, tz;
   float ms;
+  int32_t rmode;            // refresh mode of the sweeps with a refresh
+  int32_t fmode, fty, ftz;  // the fastest one-pass kernel and tile (fmode 0: none)
+  float fms;
+  int32_t ff;               // 1: single sweeps of the pair mode on that tile, 0: the default
 };
 static std::vector<TuneEntry> g_tuned;
+
+// A heat-bath table for timing only, shaped like the real ones: the byte of
+// eight independent bits, each set with probability q, with thresholds
+// 2^64 P(byte <= n) in double precision (the table search costs what it costs
+// on real tables, whose thresholds gather in few presearch buckets); every
+// presearch entry spans 0..255 (valid for msa_set_luts).
+static void timing_lut(msa_lut *t, double q) {
+  double cum = 0;
+  for (int n = 0; n < 256; ++n) {
+    const int k = __builtin_popcount((unsigned)n);
+    double p = 1;
+    for (int b = 0; b < 8; ++b) p *= b < k ? q : 1 - q;
+    cum += p;
+    const double v = cum * 18446744073709551616.0;
+    const uint64_t u = n == 255 || v >= 18446744073709551615.0 ? ~0ull : (uint64_t)v;
+    t->umbrales[2 * n] = (uint32_t)(u >> 32);
+    t->umbrales[2 * n + 1] = (uint32_t)u;
+  }
+  for (int n = 1; n < 256; ++n) {           // non-decreasing after rounding
+    const uint64_t a = ((uint64_t)t->umbrales[2 * n - 2] << 32) | t->umbrales[2 * n - 1];
+    const uint64_t b = ((uint64_t)t->umbrales[2 * n] << 32) | t->umbrales[2 * n + 1];
+    if (b < a) {
+      t->umbrales[2 * n] = t->umbrales[2 * n - 2];
+      t->umbrales[2 * n + 1] = t->umbrales[2 * n - 1];
+    }
+  }
+  for (int m = 0; m < 512; ++m) t->prebusqueda[m] = 0xFF00FF00u;
+}
+
+// Times `sweeps` sweeps with a refresh each in refresh mode `rm` on scratch
+// context t, whose refresh tables cover them.
+static int time_refresh(msa_ctx *t, int32_t sweeps, int rm, float *per) {
+  int rc = msa_set_refresh_mode(t, rm);
+  if (rc) return rc;
+  float ms = 0;
+  if ((rc = msa_run(t, 0, 4, 1))) return rc;                    // warm-up
+  if ((rc = msa_run_timed(t, 4, sweeps, 1, &ms))) return rc;
+  *per = ms / (float)sweeps;
+  return MSA_OK;
+}
+
+// The tuned refresh mode, and for the sweep-pair mode the one-pass kernel
+// and tile of its single sweeps: the fastest one-pass candidate where that
+// timed faster with the refresh, else the default (msa_set_tiling's).
+static void apply_tuned(msa_ctx *c, const TuneEntry &e) {
+  c->tuned_rmode = e.rmode;
+  if (c->pair && e.ff && e.fmode > 0) set_fused(c, e.fmode, (uint32_t)e.fty, (uint32_t)e.ftz);
+}
 
 extern "C" int msa_autotune(msa_ctx *c, int32_t sweeps, int32_t *mode, int32_t *ty, int32_t *tz,
                             float *ms_per_sweep) {
@@ -2064,6 +2164,7 @@ extern "C" int msa_autotune(msa_ctx *c, int32_t sweeps, int32_t *mode, int32_t *
     if (e.L == c->g.L && e.R == c->g.R && e.nc == c->g.nc) {
       int rc = msa_set_tiling(c, e.mode, e.ty, e.tz);
       if (rc) return rc;
+      apply_tuned(c, e);
       if (mode) *mode = e.mode;
       if (ty) *ty = e.ty;
       if (tz) *tz = e.tz;
@@ -2086,7 +2187,7 @@ extern "C" int msa_autotune(msa_ctx *c, int32_t sweeps, int32_t *mode, int32_t *
   int rc = msa_create(c->device, &c->geom, &t);
   if (rc) return rc;
   rc = msa_init_philox(t, 0xA070u);
-  TuneEntry best{c->g.L, c->g.R, c->g.nc, 0, 0, 0, 1e30f};
+  TuneEntry best{c->g.L, c->g.R, c->g.nc, 0, 0, 0, 1e30f, 0, 0, 0, 0, 1e30f, 0};
   for (const Cand &cd : cands) {
     if (rc) break;
     if ((rc = msa_set_tiling(t, cd.mode, cd.ty, cd.tz))) break;
@@ -2100,11 +2201,57 @@ extern "C" int msa_autotune(msa_ctx *c, int32_t sweeps, int32_t *mode, int32_t *
       best.tz = cd.tz;
       best.ms = per;
     }
+    if ((cd.mode == 1 || cd.mode == 2) && per < best.fms) {
+      best.fmode = cd.mode;
+      best.fty = cd.ty;
+      best.ftz = cd.tz;
+      best.fms = per;
+    }
+  }
+  // refresh sweeps (a refresh every sweep) with that tiling: drawn inside the
+  // sweep (0), drawn ahead (1), drawn ahead with D2 pairs (2)
+  if (!rc) {
+    // long enough for the draws to run ahead of the sweeps (several draws of
+    // up to 64 refreshes); the modes take turns, twice, and each keeps its
+    // faster run
+    const int32_t rsweeps = sweeps > 256 ? sweeps : 256;
+    const int32_t nref = rsweeps + 8;
+    std::vector<msa_lut> lut(2);
+    // the demon planes' bit probabilities near T = 1.4, mid-anneal
+    timing_lut(&lut[0], 0.06);
+    timing_lut(&lut[1], 0.004);
+    std::vector<msa_u32x4> ctr((size_t)nref * c->g.R);
+    for (size_t q = 0; q < ctr.size(); ++q) ctr[q] = msa_u32x4{(uint32_t)q, 7u, 0u, 11u};
+    std::vector<uint32_t> key(c->g.R, 0x5EEDu);
+    std::vector<int32_t> idx(nref, 0);
+    rc = msa_set_luts(t, 1, lut.data());
+    if (!rc) rc = msa_set_refresh(t, nref, ctr.data(), key.data(), idx.data());
+    // variants: mode 0; mode 1 with the default and with the fastest
+    // one-pass tile for its sweeps; mode 2 (whose sweeps run in pairs)
+    float per_v[4] = {1e30f, 1e30f, 1e30f, 1e30f};
+    const bool two_ff = t->pair && best.fmode > 0;
+    for (int round = 0; round < 2 && !rc; ++round)
+      for (int v = 0; v < 4 && !rc; ++v) {
+        if ((v == 2 && !two_ff) || (v == 3 && t->qty == 0)) continue;
+        TuneEntry e = best;
+        e.ff = v == 2;
+        if ((rc = msa_set_tiling(t, best.mode, best.ty, best.tz))) break;
+        apply_tuned(t, e);
+        float per = 0;
+        if ((rc = time_refresh(t, rsweeps, v == 0 ? 0 : (v == 3 ? 2 : 1), &per))) break;
+        if (per < per_v[v]) per_v[v] = per;
+      }
+    int bv = 0;
+    for (int v = 1; v < 4; ++v)
+      if (per_v[v] < per_v[bv]) bv = v;
+    best.rmode = bv == 0 ? 0 : (bv == 3 ? 2 : 1);
+    best.ff = bv == 2;
   }
   msa_destroy(t);
   if (rc) return rc;
   g_tuned.push_back(best);
   if ((rc = msa_set_tiling(c, best.mode, best.ty, best.tz))) return rc;
+  apply_tuned(c, best);
   if (mode) *mode = best.mode;
   if (ty) *ty = best.ty;
   if (tz) *tz = best.tz;

@@ -229,7 +229,9 @@ class Lattice:
         return m.value, ty.value, tz.value, ms.value
 
     def set_refresh_mode(self, mode):
-        """0: refresh drawn inside the sweep; 1: drawn ahead on a second stream."""
+        """0: refresh drawn inside the sweep; 1: drawn ahead on a second stream;
+        2: as 1, two refresh sweeps in a row as one sweep pair; -1: the
+        library's choice for the geometry."""
         _check(self.lib.msa_set_refresh_mode(self.ctx, mode))
 
     def refresh_mode(self):
