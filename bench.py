@@ -267,12 +267,14 @@ class Xoshiro256pp:
 def plain_seconds(pkg, tables, reqs, jobs):
     """Device time of the plain form: each lattice alone through msa_run, one
     after the other (its setup not counted), same tables and seeds as the
-    batched and served forms."""
+    batched and served forms; the kernel, tile and refresh mode of each
+    lattice size are the autotuner's (tuned once per size)."""
     total = 0.0
     tick = Ticker()
     for n, (req, job) in enumerate(zip(reqs, jobs)):
         tick("plain form: %d of %d lattices" % (n, len(reqs)))
         lat = pkg.Lattice(req.L, req.R, 1)
+        lat.autotune(16)
         lat.init_philox(req.seed)
         lat.set_luts(tables.luts)
         ctr = np.ctypeslib.as_array(job.ctr, shape=(job.nref * 4 * req.R,)).copy()

@@ -749,6 +749,17 @@ int main(int argc, char **argv)
   msa_geom g = {L, R, in.nc};
   msa_ctx *ctx = NULL;
   CHK(msa_create(device, &g, &ctx));
+  {
+    /* the sweep kernel, its tile and the refresh mode for this geometry,
+     * timed on a scratch lattice (every choice gives the same bits) */
+    int32_t km = 0, kty = 0, ktz = 0, rmode = 0;
+    float kms = 0;
+    CHK(msa_autotune(ctx, 16, &km, &kty, &ktz, &kms));
+    CHK(msa_get_refresh_mode(ctx, &rmode));
+    printf("Kernel: mode %d, tile %d x %d, %.4f ms per sweep without refresh; refresh mode %d\n",
+           km, kty, ktz, kms, rmode);
+    fflush(stdout);
+  }
   CHK(msa_upload_state(ctx, J, S, flag == 0 ? D0 : NULL, flag == 0 ? D1 : NULL));
 
   if (in.nterm > 0 && (flag == 1 || refmode)) {
