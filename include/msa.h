@@ -288,6 +288,11 @@ int msa_copy_probe(int device, int64_t bytes, int32_t iters, float *ms_per_copy)
 /* Wait for all work on the context stream. */
 int msa_sync(msa_ctx *ctx);
 
+/* Without waiting: *done = 1 when all work queued on the context stream has
+ * run (what msa_sync would wait for), else 0.  No reference counterpart: for
+ * host loops that drive several lattices (the serving loop). */
+int msa_query(msa_ctx *ctx, int32_t *done);
+
 /* Bytes of device memory held by a context of this geometry (for admission
  * by HBM footprint). */
 int64_t msa_footprint_bytes(const msa_geom *geom);

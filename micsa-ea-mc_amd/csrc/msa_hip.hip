@@ -2266,6 +2266,19 @@ extern "C" int msa_sync(msa_ctx *c) {
   return MSA_OK;
 }
 
+extern "C" int msa_query(msa_ctx *c, int32_t *done) {
+  if (!c || !done) return msa_fail(MSA_ERR_ARG, "NULL argument");
+  HIPCHK(hipSetDevice(c->device));
+  const hipError_t e = hipStreamQuery(c->stream);
+  if (e == hipErrorNotReady) {
+    *done = 0;
+    return MSA_OK;
+  }
+  HIPCHK(e);
+  *done = 1;
+  return MSA_OK;
+}
+
 extern "C" const char *msa_strerror(int status) {
   if (status == MSA_OK) return "ok";
   return g_err;

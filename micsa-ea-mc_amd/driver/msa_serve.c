@@ -4,7 +4,7 @@
  * process, PROGRAM/src/main.cu:33-50).
  *
  * usage: msa_serve requests.txt results.txt [--max-bytes N] [--epoch N]
- *                  [--block B] [--workers W] [--state-dir DIR]
+ *                  [--block B] [--workers W] [--state-dir DIR] [--solo-min-L N]
  *
  * requests.txt   one request per line: rid L R niter T0 T1 seed arrival_s
  *                (arrival in seconds after the start of the stream; '#' starts
@@ -26,12 +26,23 @@
  * --state-dir    write each request's final spins and demons (canonical order,
  *                spins [R][V] then d0 and d1 [R][V], uint64) to
  *                DIR/req_<rid>.bin
+ * --solo-min-L   requests with L >= N run one at a time as plain lattices
+ *                (msa_run with the autotuner's kernel and refresh mode, the
+ *                same grid tables) beside the batch, which takes the others;
+ *                0 = every request in the batch (default 128: at L = 128
+ *                the plain form, its refresh drawn ahead, runs 1.4 times the
+ *                batch kernel's rate; at L = 96 the two are within 5 %)
  *
  * A request is admitted, in arrival order, once it has arrived, its tables
  * are built and its footprint fits next to the lattices on the device; every
  * admitted lattice advances in the same launches (msa_batch_*); one epoch
  * stays queued on the device while the previous epoch's finished requests are
- * measured and retired.  Prints a JSON summary line on stdout.
+ * measured and retired.  Solo requests (--solo-min-L) wait in their own FIFO
+ * and run on a plain context per lattice geometry (built, autotuned and given
+ * the grid tables before the stream starts), in chunks of iterations queued
+ * whenever the previous chunk has run, so the loop keeps serving the batch;
+ * their device memory is not counted against --max-bytes.  Prints a JSON
+ * summary line on stdout.
  */
 #include <errno.h>
 #include <math.h>
@@ -224,6 +235,51 @@ static int tune_block(const msa_lut *luts, const request *reqs, int n)
   return best;
 }
 
+/* ---------------------------------------------------------- solo lane -- */
+/* Plain contexts of the solo requests, one per (L, R), reused. */
+typedef struct {
+  int L, R;
+  msa_ctx *ctx;
+} solo_ctx;
+
+#define SOLO_CHUNK 2048   /* iterations queued at a time */
+
+static msa_ctx *solo_ctx_of(solo_ctx *sc, int nsc, int L, int R)
+{
+  for (int k = 0; k < nsc; k++)
+    if (sc[k].L == L && sc[k].R == R) return sc[k].ctx;
+  return NULL;
+}
+
+/* A finished request's per-lane energies, its state file and its results
+ * line; from its plain context (ctx) or its batch slot. */
+static void write_result(FILE *out, request *q, const char *state_dir, msa_ctx *ctx, msa_batch *b,
+                         int slot)
+{
+  int32_t *unsat = q->energy;
+  if (ctx) CHK(msa_measure(ctx, unsat, NULL, NULL, NULL));
+  else CHK(msa_batch_measure(b, slot, unsat, NULL, NULL, NULL));
+  const int64_t V = (int64_t)q->L * q->L * q->L;
+  for (int i = 0; i < q->R * 64; i++) unsat[i] = (int32_t)(3 * V - 2 * (int64_t)unsat[i]);
+  if (state_dir) {
+    uint64_t *st = (uint64_t *)malloc(sizeof(uint64_t) * 3 * (size_t)(q->R * V));
+    if (!st) fail("out of memory");
+    if (ctx) CHK(msa_download_state(ctx, st, st + q->R * V, st + 2 * q->R * V));
+    else CHK(msa_batch_download_state(b, slot, st, st + q->R * V, st + 2 * q->R * V));
+    char path[4096];
+    snprintf(path, sizeof path, "%s/req_%d.bin", state_dir, q->rid);
+    FILE *sf = fopen(path, "wb");
+    if (!sf || fwrite(st, sizeof(uint64_t), 3 * (size_t)(q->R * V), sf) != 3 * (size_t)(q->R * V))
+      fail("cannot write %s", path);
+    fclose(sf);
+    free(st);
+  }
+  fprintf(out, "%d %d %d %lld %.6f %.6f %.6f %.6f", q->rid, q->L, q->R, (long long)q->niter,
+          q->arrival, q->admit, q->finish, q->finish - q->arrival);
+  for (int i = 0; i < q->R * 64; i++) fprintf(out, " %d", q->energy[i]);
+  fprintf(out, "\n");
+}
+
 /* ------------------------------------------------------------------ main -- */
 static int cmp_double(const void *a, const void *b)
 {
@@ -256,12 +312,13 @@ int main(int argc, char **argv)
 {
   if (argc < 3) {
     fprintf(stderr, "usage: %s requests.txt results.txt [--max-bytes N] [--epoch N] [--block B] "
-                    "[--workers W] [--state-dir DIR]\n", argv[0]);
+                    "[--workers W] [--state-dir DIR] [--solo-min-L N]\n", argv[0]);
     return 1;
   }
   int64_t max_bytes = (int64_t)8 << 30;
   int epoch = 64, block = 0, workers = 4;
   const char *state_dir = NULL;
+  int solo_min_L = 128;
   for (int a = 3; a < argc; a++) {
     const char *opt = argv[a];
     if (a + 1 >= argc) fail("option %s needs a value", opt);
@@ -271,9 +328,10 @@ int main(int argc, char **argv)
     else if (!strcmp(opt, "--block")) block = atoi(v);
     else if (!strcmp(opt, "--workers")) workers = atoi(v);
     else if (!strcmp(opt, "--state-dir")) state_dir = v;
+    else if (!strcmp(opt, "--solo-min-L")) solo_min_L = atoi(v);
     else fail("unknown option %s", opt);
   }
-  if (epoch < 1 || workers < 1 || max_bytes < 1) fail("bad option value");
+  if (epoch < 1 || workers < 1 || max_bytes < 1 || solo_min_L < 0) fail("bad option value");
 
   /* requests */
   FILE *f = fopen(argv[1], "r");
@@ -333,6 +391,27 @@ int main(int argc, char **argv)
   CHK(msa_batch_set_luts(b, g_nlut, luts));
   CHK(msa_batch_set_shape(b, block, epoch));
 
+  /* the solo lane's contexts, ready before the stream starts */
+  solo_ctx sc[64];
+  int nsc = 0;
+  for (int i = 0; i < n && solo_min_L > 0; i++) {
+    const request *q = &reqs[i];
+    if (q->L < solo_min_L || solo_ctx_of(sc, nsc, q->L, q->R)) continue;
+    if (nsc == 64) fail("more than 64 solo lattice geometries");
+    msa_geom g = {q->L, q->R, 1};
+    sc[nsc].L = q->L;
+    sc[nsc].R = q->R;
+    CHK(msa_create(0, &g, &sc[nsc].ctx));
+    CHK(msa_autotune(sc[nsc].ctx, 16, NULL, NULL, NULL, NULL));
+    CHK(msa_set_luts(sc[nsc].ctx, g_nlut, luts));
+    nsc++;
+  }
+  int *sq = (int *)malloc(sizeof(int) * (size_t)n), sqh = 0, sqt = 0;
+  int solo_cur = -1;
+  int64_t solo_enq = 0;
+  msa_ctx *solo = NULL;
+  if (!sq) fail("out of memory");
+
   pool p;
   memset(&p, 0, sizeof p);
   pthread_mutex_init(&p.mu, NULL);
@@ -363,8 +442,43 @@ int main(int argc, char **argv)
     const double now = now_s() - t_start;
     while (nxt < n && reqs[nxt].arrival <= now) {
       submit(&p, nxt);
-      queue[qt++] = nxt;
+      if (solo_min_L > 0 && reqs[nxt].L >= solo_min_L) sq[sqt++] = nxt;
+      else queue[qt++] = nxt;
       nxt++;
+    }
+    /* the solo lane: finish the running request, queue its next chunk, or
+     * start the next one */
+    if (solo_cur >= 0) {
+      request *q = &reqs[solo_cur];
+      int32_t idle = 0;
+      CHK(msa_query(solo, &idle));
+      if (idle && solo_enq < q->niter) {
+        const int64_t c = q->niter - solo_enq < SOLO_CHUNK ? q->niter - solo_enq : SOLO_CHUNK;
+        CHK(msa_run(solo, (uint64_t)solo_enq, c, 1));
+        solo_enq += c;
+      } else if (idle) {
+        q->finish = now_s() - t_start;
+        write_result(out, q, state_dir, solo, NULL, -1);
+        free_tables(q);
+        done++;
+        updates += 64.0 * q->R * (double)q->L * q->L * q->L * (double)q->niter;
+        solo_cur = -1;
+      }
+    }
+    if (solo_cur < 0 && sqh < sqt) {
+      request *q = &reqs[sq[sqh]];
+      const int rd = ready_of(q);
+      if (rd < 0) fail("request %d: refresh tables could not be built", q->rid);
+      if (rd > 0) {
+        solo = solo_ctx_of(sc, nsc, q->L, q->R);
+        CHK(msa_init_philox(solo, q->seed));
+        CHK(msa_set_refresh(solo, (int32_t)q->nref, q->ctr, q->keys, q->idx));
+        q->admit = now_s() - t_start;
+        solo_cur = sq[sqh++];
+        const int64_t c = q->niter < SOLO_CHUNK ? q->niter : SOLO_CHUNK;
+        CHK(msa_run(solo, 0, c, 1));
+        solo_enq = c;
+      }
     }
     while (qh < qt) {                                /* FIFO admission by footprint */
       request *q = &reqs[queue[qh]];
@@ -412,7 +526,9 @@ int main(int argc, char **argv)
       running += !fin;
     }
     if (!running && !nfin) {
-      if (nxt < n) {
+      if (solo_cur >= 0 || sqh < sqt) {
+        usleep(200);                               /* the solo lane runs */
+      } else if (nxt < n) {
         const double wait = reqs[nxt].arrival - (now_s() - t_start);
         if (wait > 0) usleep((useconds_t)(1e6 * (wait < 0.01 ? wait : 0.01)));
       }
@@ -437,32 +553,13 @@ int main(int argc, char **argv)
     for (int k = 0; k < nfin; k++) {                 /* retire */
       const int s = finishing[k];
       request *q = &reqs[slot_req[s]];
-      int32_t *unsat = q->energy;
-      CHK(msa_batch_measure(b, s, unsat, NULL, NULL, NULL));
-      const int64_t V = (int64_t)q->L * q->L * q->L;
-      for (int i = 0; i < q->R * 64; i++) unsat[i] = (int32_t)(3 * V - 2 * (int64_t)unsat[i]);
-      if (state_dir) {
-        uint64_t *st = (uint64_t *)malloc(sizeof(uint64_t) * 3 * (size_t)(q->R * V));
-        if (!st) fail("out of memory");
-        CHK(msa_batch_download_state(b, s, st, st + q->R * V, st + 2 * q->R * V));
-        char path[4096];
-        snprintf(path, sizeof path, "%s/req_%d.bin", state_dir, q->rid);
-        FILE *sf = fopen(path, "wb");
-        if (!sf || fwrite(st, sizeof(uint64_t), 3 * (size_t)(q->R * V), sf) != 3 * (size_t)(q->R * V))
-          fail("cannot write %s", path);
-        fclose(sf);
-        free(st);
-      }
       q->finish = now_s() - t_start;
+      write_result(out, q, state_dir, NULL, b, s);
       CHK(msa_batch_remove(b, s));
       slot_req[s] = -1;
       nactive--;
       done++;
-      updates += 64.0 * q->R * (double)V * (double)q->niter;
-      fprintf(out, "%d %d %d %lld %.6f %.6f %.6f %.6f", q->rid, q->L, q->R, (long long)q->niter,
-              q->arrival, q->admit, q->finish, q->finish - q->arrival);
-      for (int i = 0; i < q->R * 64; i++) fprintf(out, " %d", q->energy[i]);
-      fprintf(out, "\n");
+      updates += 64.0 * q->R * (double)q->L * q->L * q->L * (double)q->niter;
       const double t = now_s();
       if (done % 1000 == 0 || t - last_note >= 30.0) {
         last_note = t;
@@ -480,6 +577,7 @@ int main(int argc, char **argv)
   pthread_mutex_unlock(&p.mu);
   for (int w = 0; w < workers; w++) pthread_join(th[w], NULL);
   CHK(msa_batch_destroy(b));
+  for (int k = 0; k < nsc; k++) CHK(msa_destroy(sc[k].ctx));
 
   double *lat = (double *)malloc(sizeof(double) * (size_t)n);
   if (!lat) fail("out of memory");
@@ -487,9 +585,10 @@ int main(int argc, char **argv)
   const double p50 = percentile(lat, n, 50), p99 = percentile(lat, n, 99);
   printf("{\"requests\": %d, \"makespan_s\": %.6f, \"spin_updates_per_ps\": %.6f, "
          "\"latency_p50_s\": %.6f, \"latency_p99_s\": %.6f, \"max_concurrent\": %d, "
-         "\"max_bytes_in_use\": %lld, \"epoch\": %d, \"block\": %d}\n",
+         "\"max_bytes_in_use\": %lld, \"epoch\": %d, \"block\": %d, \"solo_requests\": %d, "
+         "\"solo_min_L\": %d}\n",
          n, makespan, updates / makespan / 1e12, p50, p99, max_active, (long long)max_bytes_used,
-         epoch, block);
+         epoch, block, sqt, solo_min_L);
   for (int i = 0; i < n; i++) free(reqs[i].energy);
   free(reqs);
   free(luts);
@@ -502,5 +601,6 @@ int main(int argc, char **argv)
   free(newfin);
   free(tp);
   free(te);
+  free(sq);
   return 0;
 }

@@ -134,7 +134,7 @@ def run_alone(req, tables, jobid):
 
 
 def serve(requests, max_bytes, epoch=64, block=None, tables=None, keep_state=False, workers=4,
-          progress=None):
+          progress=None, solo_min_L=128):
     """Run the stream through the native request loop (bin/msa_serve, the
     C driver of the served form); returns (per-request dicts, summary dict).
     block=None lets the driver's autotuner pick the batch kernel's block size
@@ -143,7 +143,9 @@ def serve(requests, max_bytes, epoch=64, block=None, tables=None, keep_state=Fal
     are ready and their footprint fits under max_bytes.  keep_state also
     returns each request's final spins and demons.  progress, if given, gets
     the driver's progress lines.  `tables` is accepted for callers that hold
-    one; the driver builds the same temperature grid itself."""
+    one; the driver builds the same temperature grid itself.  Requests with
+    L >= solo_min_L run one at a time as plain lattices beside the batch
+    (0: every request in the batch)."""
     import json
     import os
     import subprocess
@@ -159,7 +161,8 @@ def serve(requests, max_bytes, epoch=64, block=None, tables=None, keep_state=Fal
                         % (r.rid, r.L, r.R, r.niter, r.T0, r.T1, r.seed, r.arrival))
         res = os.path.join(d, "results.txt")
         cmd = [SERVE_BIN, rq, res, "--max-bytes", str(int(max_bytes)), "--epoch", str(epoch),
-               "--block", str(block or 0), "--workers", str(workers)]
+               "--block", str(block or 0), "--workers", str(workers),
+               "--solo-min-L", str(int(solo_min_L))]
         if keep_state:
             os.makedirs(os.path.join(d, "state"))
             cmd += ["--state-dir", os.path.join(d, "state")]

@@ -74,7 +74,7 @@ def test_serving_mix_equals_alone():
     limit = 3 * max(sizes)
     assert sum(sizes) > limit
     done, summary = serve.serve(reqs, max_bytes=limit, epoch=16, block=128, tables=tables,
-                                keep_state=True)
+                                keep_state=True, solo_min_L=0)
     assert summary["requests"] == len(reqs)
     assert summary["max_bytes_in_use"] <= limit          # admission by footprint held
     assert summary["max_concurrent"] >= 2                 # requests overlapped on the device
@@ -106,6 +106,8 @@ def test_served_configuration_subset(orc):
     assert summary["requests"] == len(reqs)
     assert summary["max_bytes_in_use"] <= limit
     assert summary["max_concurrent"] >= 3
+    # L = 128 ran one at a time as plain lattices beside the batch
+    assert summary["solo_requests"] == sum(r.L >= 128 for r in reqs) > 0
     check_against_alone(serve, tables, reqs, done)
     small = [r for r in reqs if r.L <= 32]
     assert len(small) >= 20
@@ -116,3 +118,19 @@ def test_served_configuration_subset(orc):
         assert np.array_equal(rec["state"][1], o.D0) and np.array_equal(rec["state"][2], o.D1), \
             "request %d: demons differ from the oracle" % r.rid
         assert np.array_equal(rec["energy"], orc.energy(o)), "request %d: energies" % r.rid
+
+
+def test_solo_lane_in_chunks():
+    """Every request on the solo lane (plain lattices one at a time, queued in
+    chunks of 2048 iterations): lengths across chunk boundaries, two lattice
+    geometries reused by later requests; each result equals the request run
+    alone."""
+    serve, msa = modules()
+    reqs = [serve.Request(i, L, n, 2.0, 0.7, 31 + i, 0.002 * i)
+            for i, (L, n) in enumerate([(16, 4100), (24, 2048), (16, 2049), (24, 700), (16, 5)])]
+    tables = serve.Tables()
+    done, summary = serve.serve(reqs, max_bytes=1 << 30, epoch=16, block=256, tables=tables,
+                                keep_state=True, solo_min_L=16)
+    assert summary["requests"] == len(reqs) and summary["solo_requests"] == len(reqs)
+    assert summary["max_concurrent"] == 0                 # nothing went to the batch
+    check_against_alone(serve, tables, reqs, done)
