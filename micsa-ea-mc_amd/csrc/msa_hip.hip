@@ -967,6 +967,45 @@ __global__ void __launch_bounds__(512) k_measure(MeasArgs a) {
 }
 
 // Counter-based construction (msa_init_philox): thread = canonical site.
+// Layout conversion of state transfers on the device: canonical site order
+// s = x + L (y + L z) (the host arrays) <-> the colour-split planes.  Thread
+// per (plane q, site s); TO_CANON: out[q V + s] = in[(2 q + col) H + h]
+// (spins and demons keep both colours of a replica / layer next to each
+// other) or in[(col nplane + q) H + h] (cmajor: couplings), else the
+// reverse.  Demons (16-byte words) go through two 8-byte arrays, d0 and d1
+// (DEM).
+template <bool TO_CANON, bool DEM>
+__global__ void __launch_bounds__(256) k_relayout(const void *in, void *out0, void *out1, Geo g,
+                                                  uint32_t nplane, int cmajor) {
+  const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (uint64_t)nplane * g.V) return;
+  const uint64_t q = gid / g.V, s = gid - q * g.V;
+  const uint32_t x = (uint32_t)(s % g.L);
+  const uint64_t yz = s / g.L;
+  const uint32_t y = (uint32_t)(yz % g.L), z = (uint32_t)(yz / g.L);
+  const uint32_t col = (x + y + z) & 1u;
+  const uint64_t h = (x >> 1) + (uint64_t)g.Lh * (y + (uint64_t)g.L * z);
+  // split index: colour minor (spins, demons) or major (couplings: plane
+  // col 3 + d)
+  const uint64_t sp = (cmajor ? (uint64_t)col * nplane + q : q * 2 + col) * g.H + h;
+  if (DEM) {
+    if (TO_CANON) {
+      const ulonglong2 v = reinterpret_cast<const ulonglong2 *>(in)[sp];
+      reinterpret_cast<uint64_t *>(out0)[gid] = v.x;
+      reinterpret_cast<uint64_t *>(out1)[gid] = v.y;
+    } else {
+      const uint64_t a = reinterpret_cast<const uint64_t *>(in)[gid];
+      const uint64_t b = reinterpret_cast<const uint64_t *>(out1)[gid];   // d1 (input)
+      reinterpret_cast<ulonglong2 *>(out0)[sp] = make_ulonglong2(a, b);
+    }
+  } else {
+    if (TO_CANON)
+      reinterpret_cast<uint64_t *>(out0)[gid] = reinterpret_cast<const uint64_t *>(in)[sp];
+    else
+      reinterpret_cast<uint64_t *>(out0)[sp] = reinterpret_cast<const uint64_t *>(in)[gid];
+  }
+}
+
 __global__ void __launch_bounds__(256) k_init_philox(uint64_t *sp, uint64_t *jp, Geo g,
                                                      uint32_t k0, uint32_t k1) {
   const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1332,42 +1371,59 @@ extern "C" int msa_get_refresh_mode(msa_ctx *c, int32_t *mode) {
   return MSA_OK;
 }
 
+// Host arrays in canonical order go through a device scratch buffer of the
+// largest array's size, allocated for the call: copied as they are, then
+// permuted by k_relayout on the device (the host does no per-site work).
+static int relayout(msa_ctx *c, bool to_canon, bool dem, const void *in, void *out0, void *out1,
+                    uint32_t nplane, int cmajor) {
+  const uint64_t n = (uint64_t)nplane * c->g.V;
+  const dim3 grid((uint32_t)((n + 255) / 256));
+  if (to_canon && dem)
+    hipLaunchKernelGGL((k_relayout<true, true>), grid, dim3(256), 0, c->stream, in, out0, out1, c->g, nplane, cmajor);
+  else if (to_canon)
+    hipLaunchKernelGGL((k_relayout<true, false>), grid, dim3(256), 0, c->stream, in, out0, out1, c->g, nplane, cmajor);
+  else if (dem)
+    hipLaunchKernelGGL((k_relayout<false, true>), grid, dim3(256), 0, c->stream, in, out0, out1, c->g, nplane, cmajor);
+  else
+    hipLaunchKernelGGL((k_relayout<false, false>), grid, dim3(256), 0, c->stream, in, out0, out1, c->g, nplane, cmajor);
+  HIPCHK(hipGetLastError());
+  return MSA_OK;
+}
+
+struct Scratch {
+  void *p = nullptr;
+  ~Scratch() { if (p) (void)hipFree(p); }
+};
+
 extern "C" int msa_upload_state(msa_ctx *c, const uint64_t *J, const uint64_t *spin,
                                 const uint64_t *d0, const uint64_t *d1) {
   if (!c || !J || !spin) return msa_fail(MSA_ERR_ARG, "NULL argument");
   if ((d0 == nullptr) != (d1 == nullptr)) return msa_fail(MSA_ERR_ARG, "d0 and d1 must both be given");
   HIPCHK(hipSetDevice(c->device));
   const Geo &g = c->g;
-  const uint64_t V = g.V, H = g.H;
-  std::vector<uint64_t> jb(3 * V), sb(g.R * V);
-  for (uint64_t s = 0; s < V; ++s) {
-    uint32_t col;
-    const uint64_t h = split_index(g, s, &col);
-    for (int d = 0; d < 3; ++d) jb[(col * 3 + d) * H + h] = J[d * V + s];
-    for (uint32_t r = 0; r < g.R; ++r) sb[(2 * r + col) * H + h] = spin[r * V + s];
-  }
+  const uint64_t V = g.V, nd = (uint64_t)g.R * g.nc * V;
+  uint64_t need = 24 * V;
+  if (8 * g.R * V > need) need = 8 * g.R * V;
+  if (d0 && 16 * nd > need) need = 16 * nd;
   HIPCHK(hipStreamSynchronize(c->stream));
-  HIPCHK(hipMemcpyAsync(c->jp, jb.data(), 8 * 3 * V, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(c->sp[0], sb.data(), 8 * g.R * V, hipMemcpyHostToDevice, c->stream));
+  Scratch x;
+  if (hipMalloc(&x.p, need) != hipSuccess) return msa_fail(MSA_ERR_NOMEM, "transfer scratch of %llu bytes", (unsigned long long)need);
+  uint64_t *xb = static_cast<uint64_t *>(x.p);
+  int rc;
+  HIPCHK(hipMemcpyAsync(xb, J, 8 * 3 * V, hipMemcpyHostToDevice, c->stream));
+  if ((rc = relayout(c, false, false, xb, c->jp, nullptr, 3, 1))) return rc;
+  HIPCHK(hipMemcpyAsync(xb, spin, 8 * g.R * V, hipMemcpyHostToDevice, c->stream));
+  if ((rc = relayout(c, false, false, xb, c->sp[0], nullptr, g.R, 0))) return rc;
   c->scur = 0;
   c->cur = 0;
   if (d0) {
-    std::vector<ulonglong2> db((uint64_t)g.R * g.nc * V);
-    for (uint64_t s = 0; s < V; ++s) {
-      uint32_t col;
-      const uint64_t h = split_index(g, s, &col);
-      for (uint32_t r = 0; r < g.R; ++r)
-        for (uint32_t l = 0; l < g.nc; ++l) {
-          const uint64_t src = ((uint64_t)r * g.nc + l) * V + s;
-          db[(((uint64_t)r * g.nc + l) * 2 + col) * H + h] = make_ulonglong2(d0[src], d1[src]);
-        }
-    }
-    HIPCHK(hipMemcpyAsync(c->dm[0], db.data(), 16 * db.size(), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpyAsync(xb, d0, 8 * nd, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(xb + nd, d1, 8 * nd, hipMemcpyHostToDevice, c->stream));
+    if ((rc = relayout(c, false, true, xb, c->dm[0], xb + nd, g.R * g.nc, 0))) return rc;
   } else {
-    HIPCHK(hipMemsetAsync(c->dm[0], 0, 16ull * g.R * g.nc * V, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemsetAsync(c->dm[0], 0, 16 * nd, c->stream));
   }
+  HIPCHK(hipStreamSynchronize(c->stream));
   return MSA_OK;
 }
 
@@ -1375,51 +1431,38 @@ extern "C" int msa_download_state(msa_ctx *c, uint64_t *spin, uint64_t *d0, uint
   if (!c) return msa_fail(MSA_ERR_ARG, "NULL context");
   HIPCHK(hipSetDevice(c->device));
   const Geo &g = c->g;
-  const uint64_t V = g.V, H = g.H;
+  const uint64_t V = g.V, nd = (uint64_t)g.R * g.nc * V;
+  uint64_t need = spin ? 8 * g.R * V : 0;
+  if ((d0 || d1) && 16 * nd > need) need = 16 * nd;
   HIPCHK(hipStreamSynchronize(c->stream));
+  if (!need) return MSA_OK;
+  Scratch x;
+  if (hipMalloc(&x.p, need) != hipSuccess) return msa_fail(MSA_ERR_NOMEM, "transfer scratch of %llu bytes", (unsigned long long)need);
+  uint64_t *xb = static_cast<uint64_t *>(x.p);
+  int rc;
   if (spin) {
-    std::vector<uint64_t> sb(g.R * V);
-    HIPCHK(hipMemcpyAsync(sb.data(), c->sp[c->scur], 8 * g.R * V, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    for (uint64_t s = 0; s < V; ++s) {
-      uint32_t col;
-      const uint64_t h = split_index(g, s, &col);
-      for (uint32_t r = 0; r < g.R; ++r) spin[r * V + s] = sb[(2 * r + col) * H + h];
-    }
+    if ((rc = relayout(c, true, false, c->sp[c->scur], xb, nullptr, g.R, 0))) return rc;
+    HIPCHK(hipMemcpyAsync(spin, xb, 8 * g.R * V, hipMemcpyDeviceToHost, c->stream));
   }
   if (d0 || d1) {
-    std::vector<ulonglong2> db((uint64_t)g.R * g.nc * V);
-    HIPCHK(hipMemcpyAsync(db.data(), c->dm[c->cur], 16 * db.size(), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    for (uint64_t s = 0; s < V; ++s) {
-      uint32_t col;
-      const uint64_t h = split_index(g, s, &col);
-      for (uint32_t r = 0; r < g.R; ++r)
-        for (uint32_t l = 0; l < g.nc; ++l) {
-          const ulonglong2 v = db[(((uint64_t)r * g.nc + l) * 2 + col) * H + h];
-          const uint64_t dst = ((uint64_t)r * g.nc + l) * V + s;
-          if (d0) d0[dst] = v.x;
-          if (d1) d1[dst] = v.y;
-        }
-    }
+    if ((rc = relayout(c, true, true, c->dm[c->cur], xb, xb + nd, g.R * g.nc, 0))) return rc;
+    if (d0) HIPCHK(hipMemcpyAsync(d0, xb, 8 * nd, hipMemcpyDeviceToHost, c->stream));
+    if (d1) HIPCHK(hipMemcpyAsync(d1, xb + nd, 8 * nd, hipMemcpyDeviceToHost, c->stream));
   }
+  HIPCHK(hipStreamSynchronize(c->stream));
   return MSA_OK;
 }
 
 extern "C" int msa_download_couplings(msa_ctx *c, uint64_t *J) {
   if (!c || !J) return msa_fail(MSA_ERR_ARG, "NULL argument");
   HIPCHK(hipSetDevice(c->device));
-  const Geo &g = c->g;
-  const uint64_t V = g.V, H = g.H;
   HIPCHK(hipStreamSynchronize(c->stream));
-  std::vector<uint64_t> jb(3 * V);
-  HIPCHK(hipMemcpyAsync(jb.data(), c->jp, 8 * 3 * V, hipMemcpyDeviceToHost, c->stream));
+  Scratch x;
+  if (hipMalloc(&x.p, 24 * c->g.V) != hipSuccess) return msa_fail(MSA_ERR_NOMEM, "transfer scratch");
+  const int rc = relayout(c, true, false, c->jp, x.p, nullptr, 3, 1);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(J, x.p, 24 * c->g.V, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  for (uint64_t s = 0; s < V; ++s) {
-    uint32_t col;
-    const uint64_t h = split_index(g, s, &col);
-    for (int d = 0; d < 3; ++d) J[d * V + s] = jb[(col * 3 + d) * H + h];
-  }
   return MSA_OK;
 }
 
