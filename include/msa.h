@@ -288,6 +288,14 @@ int msa_copy_probe(int device, int64_t bytes, int32_t iters, float *ms_per_copy)
 /* Wait for all work on the context stream. */
 int msa_sync(msa_ctx *ctx);
 
+/* Page-locked host memory (NULL on failure) for the host arrays of the
+ * state transfers: msa_upload_state / msa_download_state on such arrays run
+ * at the host link's full rate instead of through the runtime's staging
+ * copies.  No reference counterpart (its host arrays are malloc'd,
+ * ini.c:34-60); the driver keeps its checkpoint arrays here. */
+void *msa_host_alloc(int64_t bytes);
+void msa_host_free(void *p);
+
 /* Without waiting: *done = 1 when all work queued on the context stream has
  * run (what msa_sync would wait for), else 0.  No reference counterpart: for
  * host loops that drive several lattices (the serving loop). */

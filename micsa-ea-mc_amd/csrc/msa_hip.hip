@@ -2309,6 +2309,20 @@ extern "C" int msa_sync(msa_ctx *c) {
   return MSA_OK;
 }
 
+extern "C" void *msa_host_alloc(int64_t bytes) {
+  if (bytes <= 0) return nullptr;
+  void *p = nullptr;
+  if (hipHostMalloc(&p, (size_t)bytes, hipHostMallocDefault) != hipSuccess) {
+    msa_fail(MSA_ERR_NOMEM, "page-locked host allocation of %lld bytes", (long long)bytes);
+    return nullptr;
+  }
+  return p;
+}
+
+extern "C" void msa_host_free(void *p) {
+  if (p) (void)hipHostFree(p);
+}
+
 extern "C" int msa_query(msa_ctx *c, int32_t *done) {
   if (!c || !done) return msa_fail(MSA_ERR_ARG, "NULL argument");
   HIPCHK(hipSetDevice(c->device));

@@ -694,11 +694,25 @@ int main(int argc, char **argv)
   msa_seeds_from_one(seed_u, rf.seeds_C, 64);
 
   const uint64_t V = (uint64_t)L * L * L;
-  uint64_t *J = (uint64_t *)xmalloc(3 * V * sizeof(uint64_t), "couplings");
-  uint64_t *S = (uint64_t *)xmalloc((uint64_t)R * V * sizeof(uint64_t), "spins");
-  uint64_t *D0 = (uint64_t *)calloc((size_t)((uint64_t)R * in.nc * V), sizeof(uint64_t));
-  uint64_t *D1 = (uint64_t *)calloc((size_t)((uint64_t)R * in.nc * V), sizeof(uint64_t));
-  if (!D0 || !D1) die("Problems allocating host demons");
+  /* the state arrays go to and from the device at the start and at every
+   * measurement (checkpoints): page-locked where the runtime gives it, so
+   * those transfers run at the host link's rate */
+  const size_t nJ = 3 * V * sizeof(uint64_t), nS = (size_t)R * V * sizeof(uint64_t);
+  const size_t nD = (size_t)R * in.nc * V * sizeof(uint64_t);
+  uint64_t *J = (uint64_t *)xmalloc(nJ, "couplings");
+  uint64_t *S = (uint64_t *)msa_host_alloc((int64_t)nS);
+  uint64_t *D0 = S ? (uint64_t *)msa_host_alloc((int64_t)nD) : NULL;
+  uint64_t *D1 = D0 ? (uint64_t *)msa_host_alloc((int64_t)nD) : NULL;
+  const int pinned = D1 != NULL;
+  if (!pinned) {
+    msa_host_free(S);
+    msa_host_free(D0);
+    S = (uint64_t *)xmalloc(nS, "spins");
+    D0 = (uint64_t *)xmalloc(nD, "demons");
+    D1 = (uint64_t *)xmalloc(nD, "demons");
+  }
+  memset(D0, 0, nD);
+  memset(D1, 0, nD);
   if (refmode) {
     msa_build_couplings_one(L, JANUS_SEED_J, J);
     msa_build_spins_slots(L, R, seed_u, 2, S);
@@ -934,9 +948,15 @@ int main(int argc, char **argv)
   free(energy);
   free(times);
   free(J);
-  free(S);
-  free(D0);
-  free(D1);
+  if (pinned) {
+    msa_host_free(S);
+    msa_host_free(D0);
+    msa_host_free(D1);
+  } else {
+    free(S);
+    free(D0);
+    free(D1);
+  }
   free(wpr);
   free(wxo);
   free(wst);
