@@ -77,6 +77,17 @@ int msa_upload_state(msa_ctx *ctx, const uint64_t *J, const uint64_t *spin,
 int msa_download_state(msa_ctx *ctx, uint64_t *spin, uint64_t *d0, uint64_t *d1);
 int msa_download_couplings(msa_ctx *ctx, uint64_t *J);
 
+/* msa_download_state without waiting: the arrays receive the state as the
+ * work queued so far leaves it, and hold it once msa_download_wait returns;
+ * work queued after this call (msa_run ...) runs meanwhile.  The context keeps
+ * a device staging buffer of the requested arrays' size (8 R V bytes for the
+ * spins, 16 R nc V for the demons) until msa_destroy; a call waits for the
+ * copy of the call before it.  The copy overlaps the device only into
+ * page-locked arrays (msa_host_alloc).  The driver uses it for its
+ * checkpoints (copy_from_GPU_for_write, main.cu:473-490, waits). */
+int msa_download_state_async(msa_ctx *ctx, uint64_t *spin, uint64_t *d0, uint64_t *d1);
+int msa_download_wait(msa_ctx *ctx);
+
 /* Counter-based construction on the device: J word (d,s) and spin word (r,s)
  * are Philox4x32-10 outputs of counter {s, d|r, 1|2, 0} and key
  * {seed_lo, seed_hi}; demons are cleared.  No reference counterpart: the

@@ -56,6 +56,13 @@ struct msa_ctx {
   uint32_t pK = 0;                        // refreshes per half
   uint32_t ncu = 0;                       // compute units of the device
   Predraw pd;
+  // msa_download_state_async: the state relaid out to canonical order on
+  // `stream`, then copied to the host on xstream while later work runs
+  void *xfer = nullptr;
+  size_t xfer_bytes = 0;
+  hipStream_t xstream = nullptr;
+  hipEvent_t e_snap = nullptr, e_xfer = nullptr;
+  bool xfer_busy = false;                 // a copy is queued and not yet waited for
   // demon refresh from the reference's walker streams (msa_walkers.hip)
   int32_t w_n = 0;                        // streams per replica, 0 = Philox refresh
   int w_cur = 0;                          // w_st buffer holding the current states

@@ -1258,6 +1258,13 @@ extern "C" int msa_destroy(msa_ctx *c) {
     if (c->e_used[h]) (void)hipEventDestroy(c->e_used[h]);
   }
   if (c->dstream) (void)hipStreamDestroy(c->dstream);
+  if (c->xstream) {
+    (void)hipStreamSynchronize(c->xstream);
+    (void)hipStreamDestroy(c->xstream);
+  }
+  if (c->e_snap) (void)hipEventDestroy(c->e_snap);
+  if (c->e_xfer) (void)hipEventDestroy(c->e_xfer);
+  (void)hipFree(c->xfer);
   if (c->luts_owned) (void)hipFree(c->luts);
   (void)hipFree(c->ctr);
   (void)hipFree(c->key);
@@ -1450,6 +1457,55 @@ extern "C" int msa_download_state(msa_ctx *c, uint64_t *spin, uint64_t *d0, uint
     if (d1) HIPCHK(hipMemcpyAsync(d1, xb + nd, 8 * nd, hipMemcpyDeviceToHost, c->stream));
   }
   HIPCHK(hipStreamSynchronize(c->stream));
+  return MSA_OK;
+}
+
+extern "C" int msa_download_wait(msa_ctx *c) {
+  if (!c) return msa_fail(MSA_ERR_ARG, "NULL context");
+  if (!c->xfer_busy) return MSA_OK;
+  HIPCHK(hipSetDevice(c->device));
+  c->xfer_busy = false;
+  HIPCHK(hipEventSynchronize(c->e_xfer));
+  return MSA_OK;
+}
+
+// The state as the work queued so far leaves it: relaid out on the context's
+// stream into a staging buffer the context keeps (spins, then d0, then d1),
+// which a second stream copies to the host arrays while later work on the
+// context's stream proceeds.
+extern "C" int msa_download_state_async(msa_ctx *c, uint64_t *spin, uint64_t *d0, uint64_t *d1) {
+  if (!c) return msa_fail(MSA_ERR_ARG, "NULL context");
+  int rc = msa_download_wait(c);
+  if (rc) return rc;
+  HIPCHK(hipSetDevice(c->device));
+  const Geo &g = c->g;
+  const uint64_t nS = (uint64_t)g.R * g.V, nd = (uint64_t)g.R * g.nc * g.V;
+  const bool dem = d0 || d1;
+  const size_t need = 8 * ((spin ? nS : 0) + (dem ? 2 * nd : 0));
+  if (!need) return MSA_OK;
+  if (c->xfer_bytes < need) {
+    (void)hipFree(c->xfer);
+    c->xfer = nullptr;
+    c->xfer_bytes = 0;
+    if (hipMalloc(&c->xfer, need) != hipSuccess)
+      return msa_fail(MSA_ERR_NOMEM, "staging buffer of %zu bytes", need);
+    c->xfer_bytes = need;
+  }
+  if (!c->xstream && hipStreamCreateWithFlags(&c->xstream, hipStreamNonBlocking) != hipSuccess)
+    return msa_fail(MSA_ERR_HIP, "copy stream");
+  if ((!c->e_snap && hipEventCreateWithFlags(&c->e_snap, hipEventDisableTiming) != hipSuccess) ||
+      (!c->e_xfer && hipEventCreateWithFlags(&c->e_xfer, hipEventDisableTiming) != hipSuccess))
+    return msa_fail(MSA_ERR_HIP, "copy events");
+  uint64_t *xs = static_cast<uint64_t *>(c->xfer), *xd = xs + (spin ? nS : 0);
+  if (spin && (rc = relayout(c, true, false, c->sp[c->scur], xs, nullptr, g.R, 0))) return rc;
+  if (dem && (rc = relayout(c, true, true, c->dm[c->cur], xd, xd + nd, g.R * g.nc, 0))) return rc;
+  HIPCHK(hipEventRecord(c->e_snap, c->stream));
+  HIPCHK(hipStreamWaitEvent(c->xstream, c->e_snap, 0));
+  if (spin) HIPCHK(hipMemcpyAsync(spin, xs, 8 * nS, hipMemcpyDeviceToHost, c->xstream));
+  if (d0) HIPCHK(hipMemcpyAsync(d0, xd, 8 * nd, hipMemcpyDeviceToHost, c->xstream));
+  if (d1) HIPCHK(hipMemcpyAsync(d1, xd + nd, 8 * nd, hipMemcpyDeviceToHost, c->xstream));
+  HIPCHK(hipEventRecord(c->e_xfer, c->xstream));
+  c->xfer_busy = true;
   return MSA_OK;
 }
 

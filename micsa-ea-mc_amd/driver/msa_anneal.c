@@ -82,11 +82,16 @@ static void delete_running(void)
   g_locked = 0;
 }
 
+static void flush_checkpoint(int dying);
+
 /* create_error + print_and_exit (io.c:996-1014, 1114-1122): the message goes
  * to stderr and, once the sample directory exists, to <dir>error with the
- * run's data; the lock is released. */
+ * run's data; the lock is released.  A checkpoint whose files are still to be
+ * written is written first, as the reference writes its confs before it can
+ * stop. */
 static void die(const char *fmt, ...)
 {
+  flush_checkpoint(1);
   char msg[2048];
   va_list ap;
   va_start(ap, fmt);
@@ -511,6 +516,42 @@ static void write_checkpoint(const run_files *rf, unsigned long long now,
   free(jan);
 }
 
+/* A checkpoint whose state copy is queued (msa_download_state_async) and
+ * whose files are not yet written: the loop completes it after queuing the
+ * next sweeps, so the copy and the file work overlap the device, and before
+ * any exit (die, a signal, the end of the run). */
+static struct {
+  int pending;
+  msa_ctx *ctx;
+  const run_files *rf;
+  unsigned long long now, prev;
+  int flag;
+  uint64_t seed_u;
+  const uint64_t *seed_MC, *S, *d0, *d1;
+  int32_t energy[MSA_MAXR * 64];
+  msa_prng pr[MSA_MAXR];
+  msa_xo256 xo[MSA_MAXR];
+  const msa_prng *wpr;
+  const msa_xo256 *wxo;
+} g_ck;
+
+static void flush_checkpoint(int dying)
+{
+  if (!g_ck.pending) return;
+  g_ck.pending = 0;
+  const int rc = msa_download_wait(g_ck.ctx);
+  if (rc != MSA_OK) {
+    if (dying) {
+      fprintf(stderr, "the checkpoint at iteration %llu was not written: %s\n", g_ck.now,
+              msa_strerror(rc));
+      return;
+    }
+    die("msa_download_wait: %s", msa_strerror(rc));
+  }
+  write_checkpoint(g_ck.rf, g_ck.now, g_ck.prev, g_ck.flag, g_ck.seed_u, g_ck.seed_MC, g_ck.S,
+                   g_ck.d0, g_ck.d1, g_ck.energy, g_ck.pr, g_ck.xo, g_ck.wpr, g_ck.wxo);
+}
+
 /* The demon refresh of iterations 0 .. niter-1 on the device: the
  * reference's walker streams (refmode; st = their states now) with
  * get_thresholds at beta.dat, or the Philox stream's counters and LUT pairs
@@ -830,6 +871,7 @@ int main(int argc, char **argv)
       const int64_t n = (int64_t)(target - iter) < chunk ? (int64_t)(target - iter) : chunk;
       CHK(msa_run(ctx, iter, n, sch.refresh_every));
       iter += (unsigned long long)n;
+      flush_checkpoint(0);                /* the previous checkpoint's files */
       CHK(msa_sync(ctx));
       check_signal();
     }
@@ -898,12 +940,14 @@ int main(int argc, char **argv)
     fflush(stdout);
 
     if (can_ckpt) {
-      /* copy_from_GPU_for_write (main.cu:473-490), write_walkers, write_conf */
-      CHK(msa_download_state(ctx, S, D0, D1));
-      msa_prng gpr[MSA_MAXR];
-      msa_xo256 gxo[MSA_MAXR];
-      memcpy(gpr, base_pr, sizeof gpr);
-      memcpy(gxo, base_xo, sizeof gxo);
+      /* copy_from_GPU_for_write (main.cu:473-490), write_walkers and
+       * write_conf, completed once the next sweeps are queued
+       * (flush_checkpoint) */
+      CHK(msa_download_state_async(ctx, S, D0, D1));
+      msa_prng *gpr = g_ck.pr;
+      msa_xo256 *gxo = g_ck.xo;
+      memcpy(gpr, base_pr, sizeof g_ck.pr);
+      memcpy(gxo, base_xo, sizeof g_ck.xo);
       if (refmode) {
         /* the thermal generators are not drawn after the thermalization; the
          * walker generators are the streams after the last refresh */
@@ -915,8 +959,20 @@ int main(int argc, char **argv)
       } else {
         msa_gen_skip(gpr, gxo, R, msa_sched_count(&sch, (int64_t)iter) - k_start);
       }
-      write_checkpoint(&rf, iter, it > 0 ? times[it - 1] : 0, flag, seed_u, seed_MC, S, D0, D1,
-                       energy, gpr, gxo, wpr, wxo);
+      g_ck.ctx = ctx;
+      g_ck.rf = &rf;
+      g_ck.now = iter;
+      g_ck.prev = it > 0 ? times[it - 1] : 0;
+      g_ck.flag = flag;
+      g_ck.seed_u = seed_u;
+      g_ck.seed_MC = seed_MC;
+      g_ck.S = S;
+      g_ck.d0 = D0;
+      g_ck.d1 = D1;
+      memcpy(g_ck.energy, energy, sizeof(int32_t) * 64 * (size_t)R);
+      g_ck.wpr = wpr;
+      g_ck.wxo = wxo;
+      g_ck.pending = 1;
     }
     check_signal();
     if (change_times(&in, &times)) {
@@ -936,6 +992,7 @@ int main(int argc, char **argv)
     if (check_time(max_time, dt0, it < in.ntimes ? times[it] - iter : 0) || it >= in.ntimes)
       break;
   }
+  flush_checkpoint(0);
   CHK(msa_sync(ctx));
   delete_running();
   printf("Simulation I=%d, nbits=%d finished at iteration %llu\n", sample, nbits, iter);

@@ -61,6 +61,11 @@ def hip():
         L.msa_upload_state.argtypes = [vp, u64p, u64p, u64p, u64p]
         L.msa_download_state.argtypes = [vp, u64p, u64p, u64p]
         L.msa_download_couplings.argtypes = [vp, u64p]
+        L.msa_download_state_async.argtypes = [vp, u64p, u64p, u64p]
+        L.msa_download_wait.argtypes = [vp]
+        L.msa_host_alloc.argtypes = [C.c_int64]
+        L.msa_host_alloc.restype = C.c_void_p
+        L.msa_host_free.argtypes = [C.c_void_p]
         L.msa_init_philox.argtypes = [vp, C.c_uint64]
         L.msa_set_luts.argtypes = [vp, C.c_int32, C.c_void_p]
         L.msa_set_refresh.argtypes = [vp, C.c_int32, u32p, u32p, i32p]
@@ -110,6 +115,27 @@ def _check(rc):
         raise MsaError("msa error %d: %s" % (rc, hip().msa_strerror(rc).decode()))
 
 
+class HostArray:
+    """Page-locked host memory (msa_host_alloc) seen as a uint64 numpy array."""
+
+    def __init__(self, shape):
+        n = int(np.prod(shape))
+        self.lib = hip()
+        self.ptr = self.lib.msa_host_alloc(8 * n)
+        if not self.ptr:
+            raise MsaError("msa_host_alloc of %d bytes failed" % (8 * n))
+        self.array = np.ctypeslib.as_array((C.c_uint64 * n).from_address(self.ptr)).reshape(shape)
+
+    def close(self):
+        if self.ptr:
+            self.array = None
+            self.lib.msa_host_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        self.close()
+
+
 class Lattice:
     """One device lattice: L^3 sites x 64 sample lanes x R replicas, nc demon layers."""
 
@@ -151,6 +177,19 @@ class Lattice:
         D1 = np.zeros_like(D0)
         _check(self.lib.msa_download_state(self.ctx, _p(S), _p(D0), _p(D1)))
         return S, D0, D1
+
+    def download_async(self, S, D0, D1):
+        """Queue a copy of the state into S [R][V], D0, D1 [R][nc][V] (uint64,
+        any may be None) without waiting; download_wait() completes it."""
+        for a, n in ((S, self.R * self.V), (D0, self.R * self.nc * self.V),
+                     (D1, self.R * self.nc * self.V)):
+            assert a is None or (a.dtype == np.uint64 and a.size == n and a.flags.c_contiguous)
+        self._xfer = (S, D0, D1)          # kept alive until the copy is waited for
+        _check(self.lib.msa_download_state_async(self.ctx, _p(S), _p(D0), _p(D1)))
+
+    def download_wait(self):
+        _check(self.lib.msa_download_wait(self.ctx))
+        self._xfer = None
 
     def couplings(self):
         J = np.zeros((3, self.V), np.uint64)
