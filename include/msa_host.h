@@ -41,6 +41,8 @@ uint32_t msa_swb_next(msa_swb *g);
 uint64_t msa_swb_u64(msa_swb *g);
 /* _actualiza_aleatorio_HQ_escalar, random_generator.h:71-76 */
 uint64_t msa_prng_next(msa_prng *g);
+/* n draws of msa_prng_next skipped by jump-ahead (same state as stepping) */
+void     msa_prng_skip(msa_prng *g, uint64_t n);
 /* _actualiza_xoshiro256pp, random_generator.h:58-67 */
 uint64_t msa_xo256_next(msa_xo256 *g);
 /* Init_Rand_HQ_64bits, random_generator.cu:160-200 */

@@ -82,6 +82,83 @@ uint64_t msa_prng_next(msa_prng *g)
   return g->out;
 }
 
+/* The lagged sum x_m = x_{m-24} + x_{m-55} (mod 2^64) is linear: with the
+ * state (x_{m-55} .. x_{m-1}) as the basis t^0 .. t^54, x_{m-55+k} is the
+ * state weighted by the coefficients of t^k mod (t^55 - t^31 - 1).  Products
+ * of two such polynomials, reduced by t^i = t^(i-24) + t^(i-55) for i >= 55. */
+#define LF_N 55
+static void lf_mulmod(const uint64_t *a, const uint64_t *b, uint64_t *out)
+{
+  uint64_t p[2 * LF_N - 1];
+  memset(p, 0, sizeof p);
+  for (int i = 0; i < LF_N; i++)
+    if (a[i])
+      for (int j = 0; j < LF_N; j++) p[i + j] += a[i] * b[j];
+  for (int i = 2 * LF_N - 2; i >= LF_N; i--) {
+    p[i - 24] += p[i];
+    p[i - 55] += p[i];
+  }
+  memcpy(out, p, sizeof(uint64_t) * LF_N);
+}
+
+/* t * a mod (t^55 - t^31 - 1) */
+static void lf_shift(uint64_t *a)
+{
+  const uint64_t top = a[LF_N - 1];
+  memmove(a + 1, a, sizeof(uint64_t) * (LF_N - 1));
+  a[0] = top;
+  a[31] += top;
+}
+
+/* msa_prng_next applied n times, by jump-ahead for large n: the lagged sum by
+ * t^k mod its characteristic polynomial (above), the LCG v -> MUL v + 1 as an
+ * affine map raised to the n-th power; the result equals stepping (the
+ * 62 values the output needs, x_{m+n-62} .. x_{m+n-1}, are all recurrence
+ * values once n >= 7, since they then start at or after x_{m-55}). */
+void msa_prng_skip(msa_prng *g, uint64_t n)
+{
+  if (n < 64) {
+    for (uint64_t i = 0; i < n; i++) (void)msa_prng_next(g);
+    return;
+  }
+  uint64_t st[LF_N];                         /* x_{m-55} .. x_{m-1} */
+  for (int j = 0; j < LF_N; j++) st[j] = g->ring[(uint8_t)(g->p - LF_N + j)];
+  /* c = t^(n - 7) mod P: x_{m+n-62} = x_{m-55 + (n-7)} */
+  uint64_t c[LF_N], base[LF_N];
+  memset(c, 0, sizeof c);
+  memset(base, 0, sizeof base);
+  c[0] = 1;
+  base[1] = 1;
+  for (uint64_t e = n - 7; e; e >>= 1) {
+    if (e & 1) lf_mulmod(c, base, c);
+    if (e > 1) lf_mulmod(base, base, base);
+  }
+  uint64_t x[62];
+  for (int k = 0; k < 62; k++) {
+    uint64_t v = 0;
+    for (int j = 0; j < LF_N; j++) v += c[j] * st[j];
+    x[k] = v;
+    lf_shift(c);
+  }
+  /* affine LCG power: v -> A v + B */
+  uint64_t A = 1, B = 0, a = MUL, b = 1;
+  for (uint64_t e = n; e; e >>= 1) {
+    if (e & 1) {
+      B = a * B + b;
+      A = a * A;
+    }
+    b = a * b + b;
+    a = a * a;
+  }
+  g->lcg = A * g->lcg + B;
+  g->p = (uint8_t)(g->p + n);
+  g->p24 = (uint8_t)(g->p24 + n);
+  g->p55 = (uint8_t)(g->p55 + n);
+  g->p61 = (uint8_t)(g->p61 + n);
+  for (int k = 0; k < 62; k++) g->ring[(uint8_t)(g->p - 62 + k)] = x[k];
+  g->out = g->lcg + (x[61] ^ x[0]);
+}
+
 /* _actualiza_xoshiro256pp, random_generator.h:58-67 */
 uint64_t msa_xo256_next(msa_xo256 *g)
 {

@@ -365,6 +365,7 @@ def host():
         L.msa_prng_seed_hq.argtypes = [C.POINTER(Prng), C.c_uint64]
         L.msa_prng_next.argtypes = [C.POINTER(Prng)]
         L.msa_prng_next.restype = C.c_uint64
+        L.msa_prng_skip.argtypes = [C.POINTER(Prng), C.c_uint64]
         L.msa_janus_seed.argtypes = [C.POINTER(Prng), C.c_uint64]
         L.msa_janus_next.argtypes = [C.POINTER(Prng)]
         L.msa_janus_next.restype = C.c_uint64

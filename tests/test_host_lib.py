@@ -91,7 +91,62 @@ def test_janus_and_hq_streams(pkg, H):
     assert [H.msa_prng_next(C.byref(g)) for _ in want] == want
 
 
-@pytest.mark.parametrize("L,R", [(4, 1), (8, 2), (6, 3)])
+@pytest.mark.parametrize("seeding", ["hq", "janus"])
+def test_prng_skip_equals_stepping(pkg, seeding):
+    """msa_prng_skip (jump-ahead of the lagged sum and the LCG) against n
+    plain draws, on both sides of its switch to jump-ahead (n = 64) and of
+    the ring's 256-word wrap, and after earlier draws."""
+    H = pkg.host()
+    seed = H.msa_prng_seed_hq if seeding == "hq" else H.msa_janus_seed
+    for pre in (0, 5, 300):
+        for n in (0, 1, 7, 62, 63, 64, 65, 200, 255, 256, 257, 1000, 12345, 1 << 20):
+            a, b = pkg.Prng(), pkg.Prng()
+            seed(C.byref(a), 2002 + pre)
+            for _ in range(pre):
+                H.msa_prng_next(C.byref(a))
+            C.pointer(b)[0] = a
+            for _ in range(n):
+                H.msa_prng_next(C.byref(a))
+            H.msa_prng_skip(C.byref(b), n)
+            assert (a.lcg, a.out) == (b.lcg, b.out), (pre, n)
+            assert [H.msa_prng_next(C.byref(a)) for _ in range(300)] == \
+                   [H.msa_prng_next(C.byref(b)) for _ in range(300)], (pre, n)
+
+
+@pytest.mark.parametrize("L,R,nslot", [(16, 2, 2), (8, 2, 70), (4, 1, 3)])
+def test_single_seed_construction_equals_one_stream(pkg, L, R, nslot):
+    """The reference workload's construction (msa_build_couplings_one,
+    msa_build_spins_slots), built in parallel from jumped-ahead generators,
+    against the same streams drawn one by one."""
+    H = pkg.host()
+    V = L ** 3
+    H.msa_build_couplings_one.argtypes = [C.c_int, C.c_uint64, C.c_void_p]
+    H.msa_build_spins_slots.argtypes = [C.c_int, C.c_int, C.c_uint64, C.c_int, C.c_void_p]
+    J = np.zeros((3, V), np.uint64)
+    S = np.zeros((R, V), np.uint64)
+    H.msa_build_couplings_one(L, 9164166177945067407, J.ctypes.data)
+    H.msa_build_spins_slots(L, R, 2002, nslot, S.ctypes.data)
+    g = pkg.Prng()
+    H.msa_janus_seed(C.byref(g), 9164166177945067407)
+    want = np.zeros_like(J)
+    for s in range(V):
+        for d in range(3):
+            want[d, s] = 2 ** 64 - 1 if H.msa_prng_next(C.byref(g)) >> 63 else 0
+    assert np.array_equal(J, want)
+    H.msa_prng_seed_hq(C.byref(g), 2002)
+    want = np.zeros_like(S)
+    for r in range(R):
+        for s in range(V):
+            w = 0
+            for b in range(nslot):
+                bit = H.msa_prng_next(C.byref(g)) >> 63
+                if b < 64:
+                    w |= bit << b
+            want[r, s] = w
+    assert np.array_equal(S, want)
+
+
+@pytest.mark.parametrize("L,R", [(4, 1), (8, 2), (6, 3), (16, 2), (24, 1)])
 def test_state_construction_matches_oracle(pkg, orc, L, R):
     J, S = pkg.build_state(L, R, 1001, 2002)
     o = OLattice(L, R, 1)
