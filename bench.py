@@ -58,7 +58,9 @@ def load_package():
     return mod
 
 
-KERNELS = {0: "k_halfsweep", 1: "k_sweep_fused<%d>", 2: "k_sweep_fused2<%d>", 3: "k_sweep_pair"}
+KERNELS = {0: "k_halfsweep", 1: "k_sweep_fused<%d>", 2: "k_sweep_fused2<%d>", 3: "k_sweep_pair",
+           4: "k_sweep_resident"}
+RES_MAX = 64          # iterations per resident launch (mode 4)
 
 
 def kernel_name(mode, R):
@@ -562,8 +564,10 @@ def main():
     sec = wall + missing
     updates = 64.0 * R * V * args.steps
     value = updates / sec / 1e12
-    per_sweep = {0: 2.0, 1: 1.0, 2: 1.0, 3: 0.5}[tiling[0]] * nc
-    launches_k = int(per_sweep * ksweeps)
+    if tiling[0] == 4:
+        launches_k = -(-ksweeps // RES_MAX)
+    else:
+        launches_k = int({0: 2.0, 1: 1.0, 2: 1.0, 3: 0.5}[tiling[0]] * nc * ksweeps)
     kbytes = alg_bytes(V, R, ksweeps, 0) / launches_k
     avg_s = ms_k * 1e-3 / launches_k
     achieved = kbytes / avg_s / 1e9
@@ -593,7 +597,8 @@ def main():
             "sweeps_timed": args.steps, "refreshes_timed": nref,
             "measure_every": me, "measurements_timed": nmeas,
             "kernel_mode": ["two-pass", "one-pass", "one-pass, single-load",
-                            "two sweeps per pass"][tiling[0]],
+                            "two sweeps per pass",
+                            "resident: up to %d sweeps per launch, state in LDS" % RES_MAX][tiling[0]],
             "tile_rows_planes": [tiling[1], tiling[2]],
             "refresh_mode": ["drawn inside the sweep", "drawn ahead on a second stream",
                              "drawn ahead, two refresh sweeps per pass"][rmode],
@@ -630,6 +635,11 @@ def main():
                                    "not part of value"},
         "mean_energy_per_site": round(float(-m["energy"].mean()) / V, 6),
     }
+    if tiling[0] == 4:
+        out["roofline"]["note"] = ("resident launches keep the state in LDS across their sweeps: "
+                                   "achieved counts the per-sweep compulsory bytes of the HBM "
+                                   "forms, not HBM traffic; this lattice is bound by LDS and "
+                                   "barriers, not HBM")
     if tuned is not None:
         out["autotune_ms_per_sweep"] = round(tuned[3], 5)
     if thermal is not None:

@@ -194,7 +194,13 @@ int msa_get_walker_streams(msa_ctx *ctx, msa_walker_stream *st);
  * refresh-free iterations per kernel (temporal blocking) on ty x tz tiles
  * (ty + 6 thread rows, one replica per block); a single refresh-free
  * iteration left between refreshes uses the mode 2 kernel (or mode 0 where
- * mode 2 does not exist for R).  Sweeps with a refresh always use the
+ * mode 2 does not exist for R).  mode 4 (lattices whose state fits one CU:
+ * L^3/2 <= 2048 sites of a colour and (8 + 16 nc (+ 12 when L^3/2 > 1024))
+ * L^3 bytes <= 160 KiB: L = 16 with one layer, 14 up to two, 12 up to five,
+ * 10 and below up to six; the default for them): up to 64 iterations per launch, one
+ * block per replica with the lattice in LDS; refresh sweeps whose demons are
+ * drawn ahead (refresh modes 1 and 2) run in the same launches.  Sweeps with
+ * a refresh drawn inside the sweep (refresh mode 0) always use the
  * two-kernel form.  Results are identical in every mode; only the speed
  * differs. */
 int msa_set_tiling(msa_ctx *ctx, int32_t mode, int32_t ty, int32_t tz);

@@ -42,6 +42,7 @@ struct msa_ctx {
   uint32_t fty = 0, ftz = 0, fny = 0;     // its tile: rows, planes, thread rows
   int fyhalo = 1;
   int pair = 0;                           // refresh-free sweeps two per pass (k_sweep_pair)
+  int res = 0;                            // iterations in LDS-resident launches (k_sweep_resident)
   uint32_t pty = 0, ptz = 0;              // its tile: rows, planes
   // Philox refresh sweeps: 0 = the refresh fused into the two-pass kernel's
   // demon load; 1 = demons drawn ahead by k_refresh_draw on a second stream

@@ -838,6 +838,234 @@ __global__ void __launch_bounds__(D2 ? 512 : 1024) k_sweep_pair(PairArgs a) {
   }
 }
 
+// ------------------------------------------------- resident small lattices --
+// A lattice whose spins and demons fit in one CU's LDS (8 V + 16 nc V bytes,
+// and 12 V more for the colour-1 bond planes when a thread owns two words of
+// a colour: L = 16 with one layer, 14 with two, 12 with five, 10 with six)
+// runs up to kResMax iterations in one launch: one block per replica keeps
+// the state on chip (spins and demons in LDS, each thread's own spins and
+// bonds in registers, or half of its bonds in LDS when it owns two words),
+// and the colours of each layer follow each other behind barriers, so an
+// iteration costs two to four barriers per layer instead of launches.  An
+// iteration with a refresh reads the demons drawn ahead for it (refresh
+// modes 1 and 2), loaded one colour ahead so the load overlaps the colour
+// before.  Same updates, walk and order as k_halfsweep.
+constexpr int kResMax = 64;
+constexpr size_t kResLds = 160 * 1024;
+
+// LDS of a resident block: spins, demons and, with two words of a colour per
+// thread (L/2 L^2 > 1024), the colour-1 bond planes.
+static size_t res_lds(const Geo &g) {
+  return (8 + 16 * (size_t)g.nc + (g.H > 1024 ? 12 : 0)) * g.V;
+}
+// At most two words of each colour per thread (L/2 L^2 <= 2048: L up to 16)
+// and the state within one CU's LDS.
+static bool res_fits(const Geo &g) { return g.H <= 2048 && res_lds(g) <= kResLds; }
+
+struct ResArgs {
+  const uint64_t *sp_in;    // spins [R][2][H] before the first iteration
+  uint64_t *sp_out;         // after the last (may be sp_in)
+  const uint64_t *jp;
+  const ulonglong2 *dm_in;  // demons [R][nc][2][H]
+  ulonglong2 *dm_out;       // (may be dm_in)
+  const ulonglong2 *pre;    // demons drawn ahead: slot s at pre + s R nc V
+  Geo g;
+  uint32_t n;               // iterations, at most kResMax
+  uint32_t mLh, mL;         // div_recip reciprocals of L/2 and L
+  int dm_load;              // read dm_in (0: the first iteration's refresh replaces them)
+  int dm_store;             // write dm_out (0: the next iteration's refresh replaces them)
+  int16_t slot[kResMax];    // slot of each iteration's drawn demons, -1: no refresh
+};
+
+// Site h of colour col: its neighbours' indices in the other colour's plane
+// (the indexing of k_halfsweep) and its coordinates for the walk.
+struct ResNb {
+  uint32_t xp, yp, zp, xm, ym, zm, i, y, z;
+};
+__device__ __forceinline__ ResNb res_nb(const Geo &g, int col, uint32_t h, uint32_t mLh,
+                                        uint32_t mL) {
+  const uint32_t Lh = g.Lh, L = g.L;
+  const uint32_t row = div_recip(h, mLh), i = h - row * Lh;
+  const uint32_t z = div_recip(row, mL), y = row - z * L;
+  const uint32_t off = (y + z + (uint32_t)col) & 1u;
+  const uint32_t ip = off ? msa::wrapi((int)i + 1, Lh) : i;
+  const uint32_t im = off ? i : msa::wrapi((int)i - 1, Lh);
+  const uint32_t yp = msa::wrapi((int)y + 1, L), ym = msa::wrapi((int)y - 1, L);
+  const uint32_t zp = msa::wrapi((int)z + 1, L), zm = msa::wrapi((int)z - 1, L);
+  ResNb n;
+  n.xp = ip + Lh * row;
+  n.xm = im + Lh * row;
+  n.yp = i + Lh * (yp + L * z);
+  n.ym = i + Lh * (ym + L * z);
+  n.zp = i + Lh * (y + L * zp);
+  n.zm = i + Lh * (y + L * zm);
+  n.i = i;
+  n.y = y;
+  n.z = z;
+  return n;
+}
+
+// Thread tid owns words h = tid + k blockDim (k < W) of each colour.  With
+// W = 2 the colour-1 bond planes are in LDS: a colour-0 site keeps its +x,
+// +y, +z bonds in registers (its -x, -y, -z bonds are colour-1 words), a
+// colour-1 site its -x, -y, -z bonds (colour-0 words).
+template <int W>
+struct ResState {
+  static constexpr int NJ = W == 2 ? 3 : 6;
+  uint64_t s[2][W];         // own spins
+  uint64_t J[2][W][NJ];     // own bonds: +x, +y, +z here, -x, -y, -z at the neighbours
+  ulonglong2 nx[W];         // the next colour's drawn demons
+};
+
+// The drawn demons of colour c, layer l, iteration m into st.nx.
+template <int W>
+__device__ __forceinline__ void res_fetch(const ResArgs &a, ResState<W> &st, uint32_t m,
+                                          uint32_t l, int c) {
+  const uint32_t H = (uint32_t)a.g.H, tid = threadIdx.x, bd = blockDim.x;
+  const size_t pw = (size_t)a.g.R * a.g.nc * a.g.V;
+  const ulonglong2 *p = a.pre + (size_t)a.slot[m] * pw +
+                        ((size_t)(blockIdx.x * a.g.nc + l) * 2 + (uint32_t)c) * H;
+#pragma unroll
+  for (int k = 0; k < W; ++k) {
+    const uint32_t h = tid + (uint32_t)k * bd;
+    if (h < H) st.nx[k] = p[h];
+  }
+}
+
+// Colour C of layer l in iteration m.
+template <int W, int C>
+__device__ __forceinline__ void res_colour(const ResArgs &a, ResState<W> &st, uint64_t *ls,
+                                           ulonglong2 *ld, const uint64_t *lj, uint32_t m,
+                                           uint32_t l) {
+  const Geo &g = a.g;
+  const uint32_t H = (uint32_t)g.H, tid = threadIdx.x, bd = blockDim.x;
+  const bool drawn = a.slot[m] >= 0;
+  ulonglong2 d[W];
+  ulonglong2 *plane = ld + ((size_t)l * 2 + C) * H;
+#pragma unroll
+  for (int k = 0; k < W; ++k) {
+    const uint32_t h = tid + (uint32_t)k * bd;
+    d[k] = make_ulonglong2(0, 0);
+    if (h < H) d[k] = drawn ? st.nx[k] : plane[h];
+  }
+  // the next colour's drawn demons, in flight through this colour's update
+  {
+    uint32_t m2 = m, l2 = l;
+    int c2 = C + 1;
+    if (c2 == 2) {
+      c2 = 0;
+      if (++l2 == g.nc) {
+        l2 = 0;
+        ++m2;
+      }
+    }
+    if (m2 < a.n && a.slot[m2] >= 0) res_fetch(a, st, m2, l2, c2);
+  }
+  if (!drawn) __syncthreads();          // every read of the plane before the walk writes it
+  int dx, dy, dz;
+  msa::layer_dir((int)l, dx, dy, dz);
+  const uint64_t *So = ls + (size_t)(C ^ 1) * H;
+#pragma unroll
+  for (int k = 0; k < W; ++k) {
+    const uint32_t h = tid + (uint32_t)k * bd;
+    if (h >= H) continue;
+    const ResNb nb = res_nb(g, C, h, a.mLh, a.mL);
+    uint64_t D0 = d[k].x, D1 = d[k].y;
+    const uint64_t *Jk = st.J[C][k];
+    uint64_t j[6];
+    if constexpr (W != 2) {
+#pragma unroll
+      for (int q = 0; q < 6; ++q) j[q] = Jk[q];
+    } else if constexpr (C == 0) {
+      j[0] = Jk[0];
+      j[1] = Jk[1];
+      j[2] = Jk[2];
+      j[3] = lj[nb.xm];
+      j[4] = lj[H + nb.ym];
+      j[5] = lj[2 * H + nb.zm];
+    } else {
+      j[0] = lj[h];
+      j[1] = lj[H + h];
+      j[2] = lj[2 * H + h];
+      j[3] = Jk[0];
+      j[4] = Jk[1];
+      j[5] = Jk[2];
+    }
+    const uint64_t sn = msa::site_update(st.s[C][k], So[nb.xp], So[nb.yp], So[nb.zp], So[nb.xm],
+                                         So[nb.ym], So[nb.zm], j[0], j[1], j[2], j[3], j[4], j[5],
+                                         D0, D1);
+    st.s[C][k] = sn;
+    ls[(size_t)C * H + h] = sn;
+    plane[(uint32_t)msa::walk_dest(g, C, dx, dy, dz, nb.i, nb.y, nb.z)] = make_ulonglong2(D0, D1);
+  }
+  __syncthreads();
+}
+
+template <int W>
+__global__ void __launch_bounds__(1024) k_sweep_resident(ResArgs a) {
+  extern __shared__ uint4 s_res[];
+  const Geo &g = a.g;
+  const uint32_t H = (uint32_t)g.H, nc = g.nc, tid = threadIdx.x, bd = blockDim.x;
+  const uint32_t r = blockIdx.x;
+  uint64_t *ls = reinterpret_cast<uint64_t *>(s_res);                  // spins [2][H]
+  ulonglong2 *ld = reinterpret_cast<ulonglong2 *>(ls + 2 * (size_t)H);  // demons [nc][2][H]
+  const size_t nd = (size_t)nc * 2 * H;
+  uint64_t *lj = reinterpret_cast<uint64_t *>(ld + nd);                 // W = 2: J colour 1 [3][H]
+  const uint64_t *spi = a.sp_in + 2 * (size_t)r * H;
+  for (uint32_t q = tid; q < 2 * H; q += bd) ls[q] = spi[q];
+  if constexpr (W == 2)
+    for (uint32_t q = tid; q < 3 * H; q += bd) lj[q] = a.jp[3 * (size_t)H + q];
+  if (a.dm_load) {
+    const ulonglong2 *dmi = a.dm_in + (size_t)r * nd;
+    for (uint32_t q = tid; q < nd; q += bd) ld[q] = dmi[q];
+  }
+  ResState<W> st;
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const uint64_t *Jc = a.jp + (size_t)c * 3 * H, *Jo = a.jp + (size_t)(c ^ 1) * 3 * H;
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+      const uint32_t h = tid + (uint32_t)k * bd;
+      st.s[c][k] = 0;
+#pragma unroll
+      for (int d = 0; d < ResState<W>::NJ; ++d) st.J[c][k][d] = 0;
+      if (h >= H) continue;
+      const ResNb nb = res_nb(g, c, h, a.mLh, a.mL);
+      st.s[c][k] = spi[(size_t)c * H + h];
+      if (W != 2 || c == 0) {
+        st.J[c][k][0] = Jc[h];
+        st.J[c][k][1] = Jc[H + h];
+        st.J[c][k][2] = Jc[2 * H + h];
+      }
+      if (W != 2 || c == 1) {
+        const int o = W == 2 ? 0 : 3;
+        st.J[c][k][o] = Jo[nb.xm];
+        st.J[c][k][o + 1] = Jo[H + nb.ym];
+        st.J[c][k][o + 2] = Jo[2 * H + nb.zm];
+      }
+    }
+  }
+  if (a.n > 0 && a.slot[0] >= 0) res_fetch(a, st, 0, 0, 0);
+  __syncthreads();
+  for (uint32_t m = 0; m < a.n; ++m)
+    for (uint32_t l = 0; l < nc; ++l) {
+      res_colour<W, 0>(a, st, ls, ld, lj, m, l);
+      res_colour<W, 1>(a, st, ls, ld, lj, m, l);
+    }
+  uint64_t *spo = a.sp_out + 2 * (size_t)r * H;
+#pragma unroll
+  for (int c = 0; c < 2; ++c)
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+      const uint32_t h = tid + (uint32_t)k * bd;
+      if (h < H) spo[(size_t)c * H + h] = st.s[c][k];
+    }
+  if (a.dm_store) {
+    ulonglong2 *dmo = a.dm_out + (size_t)r * nd;
+    for (uint32_t q = tid; q < nd; q += bd) dmo[q] = ld[q];
+  }
+}
+
 struct MeasArgs {
   const uint64_t *sp;     // spins [2R][H]: replica r colour c in plane 2r + c
   const uint64_t *jp;     // couplings [2 colours][3 directions][H]
@@ -1235,11 +1463,21 @@ extern "C" int msa_create(int device, const msa_geom *geom, msa_ctx **out) {
       hipFuncSetAttribute(reinterpret_cast<const void *>(&k_sweep_pair<false>),
                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPairLdsMax) != hipSuccess ||
       hipFuncSetAttribute(reinterpret_cast<const void *>(&k_sweep_pair<true>),
-                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPairLdsMax) != hipSuccess)
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPairLdsMax) != hipSuccess ||
+      hipFuncSetAttribute(reinterpret_cast<const void *>(&k_sweep_resident<1>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)kResLds) != hipSuccess ||
+      hipFuncSetAttribute(reinterpret_cast<const void *>(&k_sweep_resident<2>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)kResLds) != hipSuccess)
     return cleanup(msa_fail(MSA_ERR_HIP, "kernel LDS attribute"));
   pick_half(c);
   pick_fused(c);
   pick_pair_d2(c->g, c->qty, c->qtz);
+  // lattices that fit one CU run resident (each of the other forms is bound
+  // by its launches there)
+  if (res_fits(c->g)) {
+    c->res = 1;
+    c->fused = 0;
+  }
   if ((rc = thermal_scratch_init(device))) return cleanup(rc);
   *out = c;
   return MSA_OK;
@@ -1316,6 +1554,16 @@ static void pick_pair_d2(const Geo &g, uint32_t &bty, uint32_t &btz) {
 
 extern "C" int msa_set_tiling(msa_ctx *c, int32_t mode, int32_t ty, int32_t tz) {
   if (!c) return msa_fail(MSA_ERR_ARG, "NULL context");
+  if (mode == 4) {
+    if (!res_fits(c->g))
+      return msa_fail(MSA_ERR_ARG, "L=%u nc=%u does not fit one CU (L/2 L^2 <= 2048 and "
+                      "%zu bytes of state <= %zu)", c->g.L, c->g.nc, res_lds(c->g), kResLds);
+    c->res = 1;
+    c->pair = 0;
+    c->fused = 0;                         // refresh sweeps drawn inside the sweep: two-pass
+    return MSA_OK;
+  }
+  c->res = 0;
   if (mode == 3) {
     uint32_t y = (uint32_t)ty, z = (uint32_t)tz;
     if (ty == 0 && tz == 0 && !pick_pair(c->g, y, z))
@@ -1337,7 +1585,7 @@ extern "C" int msa_set_tiling(msa_ctx *c, int32_t mode, int32_t ty, int32_t tz) 
     return MSA_OK;
   }
   if (mode != 1 && mode != 2)
-    return msa_fail(MSA_ERR_ARG, "mode=%d (0 two-pass, 1 and 2 one-pass, 3 sweep pairs)", mode);
+    return msa_fail(MSA_ERR_ARG, "mode=%d (0 two-pass, 1 and 2 one-pass, 3 sweep pairs, 4 resident)", mode);
   if (mode == 2 && !narrow_ok(c->g))
     return msa_fail(MSA_ERR_ARG, "mode 2 needs every buffer below 4 GiB (L=%u R=%u nc=%u)", c->g.L,
                     c->g.R, c->g.nc);
@@ -1355,7 +1603,7 @@ extern "C" int msa_set_tiling(msa_ctx *c, int32_t mode, int32_t ty, int32_t tz) 
 
 extern "C" int msa_get_tiling(msa_ctx *c, int32_t *mode, int32_t *ty, int32_t *tz) {
   if (!c || !mode || !ty || !tz) return msa_fail(MSA_ERR_ARG, "NULL argument");
-  *mode = c->pair ? 3 : c->fused;
+  *mode = c->res ? 4 : (c->pair ? 3 : c->fused);
   *ty = (int32_t)(c->pair ? c->pty : c->fty);
   *tz = (int32_t)(c->pair ? c->ptz : c->ftz);
   return MSA_OK;
@@ -1646,6 +1894,31 @@ static void launch_pair(msa_ctx *c, const ulonglong2 *dmi, const ulonglong2 *dmi
   c->cur ^= 1;
 }
 
+// Iterations of a resident launch: the current buffers in, the buffers the
+// per-iteration launches would leave current out (the spin index flips nc
+// times per iteration, the demon index once).
+static void launch_resident(msa_ctx *c, ResArgs &ra) {
+  const Geo &g = c->g;
+  const uint32_t H = (uint32_t)g.H;
+  const uint32_t flips = ra.n * g.nc;
+  ra.sp_in = c->sp[c->scur];
+  ra.sp_out = c->sp[c->scur ^ (flips & 1)];
+  ra.jp = c->jp;
+  ra.dm_in = c->dm[c->cur];
+  ra.dm_out = c->dm[c->cur ^ (ra.n & 1)];
+  ra.pre = c->pre;
+  ra.g = g;
+  ra.mLh = div_recip_of(g.Lh);
+  ra.mL = div_recip_of(g.L);
+  const size_t lds = res_lds(g);
+  if (H <= 1024)
+    hipLaunchKernelGGL(k_sweep_resident<1>, dim3(g.R), dim3((H + 63) / 64 * 64), lds, c->stream, ra);
+  else
+    hipLaunchKernelGGL(k_sweep_resident<2>, dim3(g.R), dim3(1024), lds, c->stream, ra);
+  c->scur ^= flips & 1;
+  c->cur ^= ra.n & 1;
+}
+
 // The refresh mode msa_run uses (0 fused, 1 drawn ahead, 2 drawn ahead with
 // D2 pairs): the one set, or for -1 (the default) the one msa_autotune
 // measured fastest for this geometry, else demons drawn ahead where that
@@ -1813,6 +2086,45 @@ static int launch_run(msa_ctx *c, uint64_t t0, int64_t niter, int64_t every) {
   const bool d2 = pdraw && rmode == 2 && c->qty > 0;
   for (int64_t n = 0; n < niter; ++n) {
     const uint64_t t = t0 + (uint64_t)n;
+    if (c->res) {
+      // resident: the iterations from n on whose refresh, if any, is drawn
+      // ahead (a walker-stream refresh only at the first; a refresh drawn
+      // inside the sweep runs below)
+      ResArgs ra{};
+      int64_t m = n, klast = -1;
+      for (; m < niter && m - n < kResMax; ++m) {
+        const uint64_t tm = t0 + (uint64_t)m;
+        int64_t k = refresh_index(tm, every);
+        if (k >= 0 && c->w_n > 0) {
+          if (m > n) break;
+          const int rc = walkers_refresh(c, k);
+          if (rc) return rc;
+          k = -1;
+        }
+        if (k >= 0 && (!pdraw || k >= c->nref)) break;
+        ra.slot[m - n] = -1;
+        if (k >= 0) {
+          const ulonglong2 *p = nullptr;
+          const int rc = drawn_demons(c, pd, k, &p);
+          if (rc) return rc;
+          ra.slot[m - n] = (int16_t)((p - c->pre) / (ptrdiff_t)pre_words(c));
+          klast = k;
+        }
+      }
+      if (m > n) {
+        ra.n = (uint32_t)(m - n);
+        ra.dm_load = ra.slot[0] < 0;
+        const int64_t kn = m < niter ? refresh_index(t0 + (uint64_t)m, every) : -1;
+        ra.dm_store = !(pdraw && kn >= 0 && kn < ke);
+        launch_resident(c, ra);
+        if (klast >= 0) {
+          const int rc = release_before(c, pd, klast + 1);
+          if (rc) return rc;
+        }
+        n = m - 1;
+        continue;
+      }
+    }
     int64_t k = refresh_index(t, every);
     if (k >= 0 && c->w_n > 0) {           // the reference's walker streams
       const int rc = walkers_refresh(c, k);
@@ -2282,6 +2594,7 @@ extern "C" int msa_autotune(msa_ctx *c, int32_t sweeps, int32_t *mode, int32_t *
   for (uint32_t y = 1; y <= c->g.L; ++y)
     for (uint32_t z = 4; z <= c->g.L && z <= 64; ++z)
       if (pair_tile_ok(c->g, y, z)) cands.push_back({3, (int32_t)y, (int32_t)z});
+  if (res_fits(c->g)) cands.push_back({4, 0, 0});
   msa_ctx *t = nullptr;
   int rc = msa_create(c->device, &c->geom, &t);
   if (rc) return rc;
