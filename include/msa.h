@@ -160,6 +160,16 @@ int msa_run_thermal_noalloc(msa_ctx *ctx, int64_t niter, const msa_lut *lut, con
  * q and qlink have no reference counterpart.  Blocks until done. */
 int msa_measure(msa_ctx *ctx, int32_t *unsat, int32_t *demon, int32_t *q, int32_t *qlink);
 
+/* msa_measure in two halves: msa_measure_async queues the measurement of the
+ * state as the work queued so far leaves it and returns; msa_measure_wait
+ * waits for it and fills the arrays as msa_measure does.  Work queued in
+ * between (msa_run ...) runs meanwhile, so a loop can queue its next sweeps
+ * before it collects the counts.  One measurement may be queued at a time
+ * (MSA_ERR_STATE otherwise, and from msa_measure while one is queued).  No
+ * reference counterpart (Measure_Energy_Spins, main.cu:315-336, waits). */
+int msa_measure_async(msa_ctx *ctx);
+int msa_measure_wait(msa_ctx *ctx, int32_t *unsat, int32_t *demon, int32_t *q, int32_t *qlink);
+
 /* Demon refresh from the reference's walker streams (InitWalkers,
  * ini.c:712-781, run on host threads there, refresh_walkers main.cu:340-377):
  * per (sample, replica) one sequential stream of the PR+LCG and xoshiro256++

@@ -34,6 +34,9 @@ struct msa_ctx {
   int32_t ctr_cap = 0;                    // refreshes the ctr buffer holds
   std::vector<int32_t> lut_idx;           // host copy, indexes luts
   int32_t *meas = nullptr;                // counts [2R + 2(R/2)][64]
+  int32_t *meas_host = nullptr;           // page-locked copy of the counts
+  hipEvent_t e_meas = nullptr;            // after the copy of the last queued measurement
+  bool meas_busy = false;                 // a measurement is queued and not yet waited for
   void *slab = nullptr;                   // the one allocation holding sp, jp, dm and meas
   uint64_t *th_rnd = nullptr;             // thermal random words [3][V/128 * R]
   uint32_t hnr = 1, hgrid = 1;            // two-pass kernel: rows per tile, persistent blocks

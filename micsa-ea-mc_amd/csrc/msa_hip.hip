@@ -1500,6 +1500,8 @@ extern "C" int msa_destroy(msa_ctx *c) {
     (void)hipStreamSynchronize(c->xstream);
     (void)hipStreamDestroy(c->xstream);
   }
+  if (c->e_meas) (void)hipEventDestroy(c->e_meas);
+  if (c->meas_host) (void)hipHostFree(c->meas_host);
   if (c->e_snap) (void)hipEventDestroy(c->e_snap);
   if (c->e_xfer) (void)hipEventDestroy(c->e_xfer);
   (void)hipFree(c->xfer);
@@ -2410,12 +2412,26 @@ extern "C" int msa_run_thermal_noalloc(msa_ctx *c, int64_t niter, const msa_lut 
   return thermal_sweeps(c, niter, tab, dctr, dkey, rnd);
 }
 
-extern "C" int msa_measure(msa_ctx *c, int32_t *unsat, int32_t *demon, int32_t *q,
-                           int32_t *qlink) {
-  if (!c) return msa_fail(MSA_ERR_ARG, "NULL context");
+// Queue k_measure on the current state and the copy of its counts into the
+// context's page-locked array, then an event (c->e_meas) after them.
+static int measure_queue(msa_ctx *c) {
+  if (c->meas_busy)
+    return msa_fail(MSA_ERR_STATE, "a measurement is queued: msa_measure_wait first");
   HIPCHK(hipSetDevice(c->device));
   const Geo &g = c->g;
   const uint32_t nrow = meas_rows(g.R), rows = g.L * g.L;
+  const size_t nword = 64 * (size_t)nrow;
+  if (!c->meas_host) {
+    if (hipHostMalloc(reinterpret_cast<void **>(&c->meas_host), nword * sizeof(int32_t),
+                      hipHostMallocDefault) != hipSuccess) {
+      c->meas_host = nullptr;
+      return msa_fail(MSA_ERR_NOMEM, "page-locked measurement array");
+    }
+  }
+  if (!c->e_meas && hipEventCreateWithFlags(&c->e_meas, hipEventDisableTiming) != hipSuccess) {
+    c->e_meas = nullptr;
+    return msa_fail(MSA_ERR_HIP, "measurement event");
+  }
   MeasArgs a{};
   a.sp = c->sp[c->scur];
   a.jp = c->jp;
@@ -2438,19 +2454,44 @@ extern "C" int msa_measure(msa_ctx *c, int32_t *unsat, int32_t *demon, int32_t *
   a.steps = groups / fill < 1 ? 1 : (groups / fill > 16 ? 16 : groups / fill);
   const uint32_t nb = (groups + a.steps - 1) / a.steps;
   const uint32_t threads = (a.nr * g.Lh + 63) / 64 * 64;   // whole wavefronts
-  int32_t host[64 * (2 * 16 + 2 * 8)];   // R <= 16 (check_geom)
-  const size_t nword = 64 * (size_t)nrow;
   HIPCHK(hipMemsetAsync(c->meas, 0, nword * sizeof(int32_t), c->stream));
   hipLaunchKernelGGL(k_measure, dim3(nb, njob), dim3(threads), 0, c->stream, a);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(host, c->meas, nword * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpyAsync(c->meas_host, c->meas, nword * sizeof(int32_t), hipMemcpyDeviceToHost,
+                        c->stream));
+  HIPCHK(hipEventRecord(c->e_meas, c->stream));
+  c->meas_busy = true;
+  return MSA_OK;
+}
+
+extern "C" int msa_measure_async(msa_ctx *c) {
+  if (!c) return msa_fail(MSA_ERR_ARG, "NULL context");
+  return measure_queue(c);
+}
+
+extern "C" int msa_measure_wait(msa_ctx *c, int32_t *unsat, int32_t *demon, int32_t *q,
+                                int32_t *qlink) {
+  if (!c) return msa_fail(MSA_ERR_ARG, "NULL context");
+  if (!c->meas_busy) return msa_fail(MSA_ERR_STATE, "no measurement is queued");
+  HIPCHK(hipSetDevice(c->device));
+  c->meas_busy = false;
+  HIPCHK(hipEventSynchronize(c->e_meas));
+  const Geo &g = c->g;
   const size_t nR = 64 * (size_t)g.R, nP = 64 * (size_t)(g.R / 2);
+  const int32_t *host = c->meas_host;
   if (unsat) memcpy(unsat, host, nR * sizeof(int32_t));
   if (demon) memcpy(demon, host + nR, nR * sizeof(int32_t));
   if (q && nP) memcpy(q, host + 2 * nR, nP * sizeof(int32_t));
   if (qlink && nP) memcpy(qlink, host + 2 * nR + nP, nP * sizeof(int32_t));
   return MSA_OK;
+}
+
+extern "C" int msa_measure(msa_ctx *c, int32_t *unsat, int32_t *demon, int32_t *q,
+                           int32_t *qlink) {
+  if (!c) return msa_fail(MSA_ERR_ARG, "NULL context");
+  const int rc = measure_queue(c);
+  if (rc) return rc;
+  return msa_measure_wait(c, unsat, demon, q, qlink);
 }
 
 // ------------------------------------------------------- copy probe ------

@@ -72,6 +72,8 @@ def hip():
         L.msa_run.argtypes = [vp, C.c_uint64, C.c_int64, C.c_int64]
         L.msa_run_timed.argtypes = [vp, C.c_uint64, C.c_int64, C.c_int64, C.POINTER(C.c_float)]
         L.msa_measure.argtypes = [vp, i32p, i32p, i32p, i32p]
+        L.msa_measure_async.argtypes = [vp]
+        L.msa_measure_wait.argtypes = [vp, i32p, i32p, i32p, i32p]
         L.msa_sync.argtypes = [vp]
         L.msa_set_tiling.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32]
         L.msa_get_tiling.argtypes = [vp, i32p, i32p, i32p]
@@ -286,21 +288,29 @@ class Lattice:
     def sync(self):
         _check(self.lib.msa_sync(self.ctx))
 
-    def measure(self):
+    def measure(self, fn="msa_measure"):
         """Per-lane integer observables: dict of int32 arrays."""
         npair = self.R // 2
         unsat = np.zeros((self.R, 64), np.int32)
         demon = np.zeros((self.R, 64), np.int32)
         q = np.zeros((max(npair, 1), 64), np.int32)
         ql = np.zeros((max(npair, 1), 64), np.int32)
-        _check(self.lib.msa_measure(self.ctx, _p(unsat, i32p), _p(demon, i32p),
-                                    _p(q, i32p) if npair else None,
-                                    _p(ql, i32p) if npair else None))
+        _check(getattr(self.lib, fn)(self.ctx, _p(unsat, i32p), _p(demon, i32p),
+                                     _p(q, i32p) if npair else None,
+                                     _p(ql, i32p) if npair else None))
         out = {"unsat": unsat, "energy": 3 * self.V - 2 * unsat, "demon": demon}
         if npair:
             out["q"] = q[:npair]
             out["qlink"] = ql[:npair]
         return out
+
+    def measure_async(self):
+        """Queue a measurement of the state the queued work leaves (msa_measure_async)."""
+        _check(self.lib.msa_measure_async(self.ctx))
+
+    def measure_wait(self):
+        """The queued measurement's observables, as measure() returns them."""
+        return self.measure("msa_measure_wait")
 
 
 def copy_probe(nbytes, iters=20, device=0):

@@ -64,3 +64,35 @@ def test_async_download_is_a_snapshot(pkg, orc, L, R, nc, pinned):
     dev.close()
     for b in bufs:
         b.close()
+
+
+@pytest.mark.parametrize("L,R,nc", [(16, 2, 1), (24, 3, 2)])
+def test_async_measure_is_a_snapshot(pkg, orc, L, R, nc):
+    """msa_measure_async measures the state the queued work leaves, also with
+    sweeps queued before msa_measure_wait; one measurement at a time."""
+    from test_gpu_parity import assert_same_measures
+    print("case L=%d R=%d nc=%d" % (L, R, nc), flush=True)
+    o = random_state(L, R, nc, 4100 + L + R + nc)
+    dev = pkg.Lattice(L, R, nc)
+    dev.upload(o.J, o.S, o.D0, o.D1)
+    dev.run(0, 4, 0)
+    for _ in range(4):
+        orc.iteration(o)
+    dev.measure_async()
+    with pytest.raises(pkg.MsaError):
+        dev.measure_async()                 # one is queued
+    with pytest.raises(pkg.MsaError):
+        dev.measure()
+    dev.run(4, 3, 0)                        # queued before the wait: not in the counts
+    m = dev.measure_wait()
+    assert np.array_equal(m["energy"], orc.energy(o))
+    assert np.array_equal(m["demon"], orc.demon_energy(o))
+    for p in range(R // 2):
+        q, ql = orc.overlap(o, 2 * p, 2 * p + 1)
+        assert np.array_equal(m["q"][p], q) and np.array_equal(m["qlink"][p], ql)
+    with pytest.raises(pkg.MsaError):
+        dev.measure_wait()                  # nothing queued
+    for _ in range(3):
+        orc.iteration(o)
+    assert_same_measures(dev, orc, o)       # the synchronous form after it
+    dev.close()
