@@ -165,8 +165,9 @@ int msa_measure(msa_ctx *ctx, int32_t *unsat, int32_t *demon, int32_t *q, int32_
  * waits for it and fills the arrays as msa_measure does.  Work queued in
  * between (msa_run ...) runs meanwhile, so a loop can queue its next sweeps
  * before it collects the counts.  One measurement may be queued at a time
- * (MSA_ERR_STATE otherwise, and from msa_measure while one is queued).  No
- * reference counterpart (Measure_Energy_Spins, main.cu:315-336, waits). */
+ * (MSA_ERR_STATE otherwise); msa_measure in between is allowed.  The first
+ * msa_measure_async of a context makes a small page-locked array it keeps.
+ * No reference counterpart (Measure_Energy_Spins, main.cu:315-336, waits). */
 int msa_measure_async(msa_ctx *ctx);
 int msa_measure_wait(msa_ctx *ctx, int32_t *unsat, int32_t *demon, int32_t *q, int32_t *qlink);
 

@@ -2412,26 +2412,12 @@ extern "C" int msa_run_thermal_noalloc(msa_ctx *c, int64_t niter, const msa_lut 
   return thermal_sweeps(c, niter, tab, dctr, dkey, rnd);
 }
 
-// Queue k_measure on the current state and the copy of its counts into the
-// context's page-locked array, then an event (c->e_meas) after them.
-static int measure_queue(msa_ctx *c) {
-  if (c->meas_busy)
-    return msa_fail(MSA_ERR_STATE, "a measurement is queued: msa_measure_wait first");
+// Queue the clear of the counts and k_measure on the current state.
+static int measure_launch(msa_ctx *c) {
   HIPCHK(hipSetDevice(c->device));
   const Geo &g = c->g;
   const uint32_t nrow = meas_rows(g.R), rows = g.L * g.L;
   const size_t nword = 64 * (size_t)nrow;
-  if (!c->meas_host) {
-    if (hipHostMalloc(reinterpret_cast<void **>(&c->meas_host), nword * sizeof(int32_t),
-                      hipHostMallocDefault) != hipSuccess) {
-      c->meas_host = nullptr;
-      return msa_fail(MSA_ERR_NOMEM, "page-locked measurement array");
-    }
-  }
-  if (!c->e_meas && hipEventCreateWithFlags(&c->e_meas, hipEventDisableTiming) != hipSuccess) {
-    c->e_meas = nullptr;
-    return msa_fail(MSA_ERR_HIP, "measurement event");
-  }
   MeasArgs a{};
   a.sp = c->sp[c->scur];
   a.jp = c->jp;
@@ -2457,16 +2443,46 @@ static int measure_queue(msa_ctx *c) {
   HIPCHK(hipMemsetAsync(c->meas, 0, nword * sizeof(int32_t), c->stream));
   hipLaunchKernelGGL(k_measure, dim3(nb, njob), dim3(threads), 0, c->stream, a);
   HIPCHK(hipGetLastError());
+  return MSA_OK;
+}
+
+// The counts [2R + 2(R/2)][64] from `host` into the caller's arrays.
+static void measure_out(const msa_ctx *c, const int32_t *host, int32_t *unsat, int32_t *demon,
+                        int32_t *q, int32_t *qlink) {
+  const size_t nR = 64 * (size_t)c->g.R, nP = 64 * (size_t)(c->g.R / 2);
+  if (unsat) memcpy(unsat, host, nR * sizeof(int32_t));
+  if (demon) memcpy(demon, host + nR, nR * sizeof(int32_t));
+  if (q && nP) memcpy(q, host + 2 * nR, nP * sizeof(int32_t));
+  if (qlink && nP) memcpy(qlink, host + 2 * nR + nP, nP * sizeof(int32_t));
+}
+
+// A queued measurement's counts go to a page-locked array of the context
+// (made on the first such call), an event after their copy.  The device
+// counts are cleared and rewritten only by later work on the same stream,
+// so a synchronous msa_measure in between leaves the queued copy intact.
+extern "C" int msa_measure_async(msa_ctx *c) {
+  if (!c) return msa_fail(MSA_ERR_ARG, "NULL context");
+  if (c->meas_busy)
+    return msa_fail(MSA_ERR_STATE, "a measurement is queued: msa_measure_wait first");
+  HIPCHK(hipSetDevice(c->device));
+  const size_t nword = 64 * (size_t)meas_rows(c->g.R);
+  if (!c->meas_host &&
+      hipHostMalloc(reinterpret_cast<void **>(&c->meas_host), nword * sizeof(int32_t),
+                    hipHostMallocDefault) != hipSuccess) {
+    c->meas_host = nullptr;
+    return msa_fail(MSA_ERR_NOMEM, "page-locked measurement array");
+  }
+  if (!c->e_meas && hipEventCreateWithFlags(&c->e_meas, hipEventDisableTiming) != hipSuccess) {
+    c->e_meas = nullptr;
+    return msa_fail(MSA_ERR_HIP, "measurement event");
+  }
+  const int rc = measure_launch(c);
+  if (rc) return rc;
   HIPCHK(hipMemcpyAsync(c->meas_host, c->meas, nword * sizeof(int32_t), hipMemcpyDeviceToHost,
                         c->stream));
   HIPCHK(hipEventRecord(c->e_meas, c->stream));
   c->meas_busy = true;
   return MSA_OK;
-}
-
-extern "C" int msa_measure_async(msa_ctx *c) {
-  if (!c) return msa_fail(MSA_ERR_ARG, "NULL context");
-  return measure_queue(c);
 }
 
 extern "C" int msa_measure_wait(msa_ctx *c, int32_t *unsat, int32_t *demon, int32_t *q,
@@ -2476,22 +2492,21 @@ extern "C" int msa_measure_wait(msa_ctx *c, int32_t *unsat, int32_t *demon, int3
   HIPCHK(hipSetDevice(c->device));
   c->meas_busy = false;
   HIPCHK(hipEventSynchronize(c->e_meas));
-  const Geo &g = c->g;
-  const size_t nR = 64 * (size_t)g.R, nP = 64 * (size_t)(g.R / 2);
-  const int32_t *host = c->meas_host;
-  if (unsat) memcpy(unsat, host, nR * sizeof(int32_t));
-  if (demon) memcpy(demon, host + nR, nR * sizeof(int32_t));
-  if (q && nP) memcpy(q, host + 2 * nR, nP * sizeof(int32_t));
-  if (qlink && nP) memcpy(qlink, host + 2 * nR + nP, nP * sizeof(int32_t));
+  measure_out(c, c->meas_host, unsat, demon, q, qlink);
   return MSA_OK;
 }
 
 extern "C" int msa_measure(msa_ctx *c, int32_t *unsat, int32_t *demon, int32_t *q,
                            int32_t *qlink) {
   if (!c) return msa_fail(MSA_ERR_ARG, "NULL context");
-  const int rc = measure_queue(c);
+  const int rc = measure_launch(c);
   if (rc) return rc;
-  return msa_measure_wait(c, unsat, demon, q, qlink);
+  int32_t host[64 * (2 * 16 + 2 * 8)];   // R <= 16 (check_geom)
+  const size_t nword = 64 * (size_t)meas_rows(c->g.R);
+  HIPCHK(hipMemcpyAsync(host, c->meas, nword * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  measure_out(c, host, unsat, demon, q, qlink);
+  return MSA_OK;
 }
 
 // ------------------------------------------------------- copy probe ------

@@ -81,9 +81,8 @@ def test_async_measure_is_a_snapshot(pkg, orc, L, R, nc):
     dev.measure_async()
     with pytest.raises(pkg.MsaError):
         dev.measure_async()                 # one is queued
-    with pytest.raises(pkg.MsaError):
-        dev.measure()
     dev.run(4, 3, 0)                        # queued before the wait: not in the counts
+    between = dev.measure()                 # a synchronous one in between leaves it intact
     m = dev.measure_wait()
     assert np.array_equal(m["energy"], orc.energy(o))
     assert np.array_equal(m["demon"], orc.demon_energy(o))
@@ -94,5 +93,6 @@ def test_async_measure_is_a_snapshot(pkg, orc, L, R, nc):
         dev.measure_wait()                  # nothing queued
     for _ in range(3):
         orc.iteration(o)
+    assert np.array_equal(between["energy"], orc.energy(o))
     assert_same_measures(dev, orc, o)       # the synchronous form after it
     dev.close()
