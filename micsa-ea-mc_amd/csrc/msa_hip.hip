@@ -853,9 +853,14 @@ __global__ void __launch_bounds__(D2 ? 512 : 1024) k_sweep_pair(PairArgs a) {
 constexpr int kResMax = 64;
 constexpr size_t kResLds = 160 * 1024;
 
-// LDS of a resident block: spins, demons and, with two words of a colour per
-// thread (L/2 L^2 > 1024), the colour-1 bond planes.
+// One layer and x rows that tile a wavefront: the demons stay in registers
+// (k_sweep_resident_x).
+static bool res_x(const Geo &g) { return g.nc == 1 && 64 % g.Lh == 0; }
+// LDS of a resident block: spins and all bond planes (res_x); else spins,
+// demons and, with two words of a colour per thread (L/2 L^2 > 1024), the
+// colour-1 bond planes.
 static size_t res_lds(const Geo &g) {
+  if (res_x(g)) return 32 * (size_t)g.V;
   return (8 + 16 * (size_t)g.nc + (g.H > 1024 ? 12 : 0)) * g.V;
 }
 // At most two words of each colour per thread (L/2 L^2 <= 2048: L up to 16)
@@ -1064,6 +1069,128 @@ __global__ void __launch_bounds__(1024) k_sweep_resident(ResArgs a) {
     ulonglong2 *dmo = a.dm_out + (size_t)r * nd;
     for (uint32_t q = tid; q < nd; q += bd) dmo[q] = ld[q];
   }
+}
+
+// The resident form for one demon layer and x rows that tile a wavefront
+// (L/2 divides 64: L = 4, 8, 16).  Layer 0's demons walk one word along x
+// inside their row, so each thread keeps its words' demons in registers and
+// the walk is a lane shift within the wavefront (ds_bpermute): no demon
+// planes in LDS, one barrier per colour.  The LDS holds the spins and all six
+// bond planes (32 V bytes).
+__device__ __forceinline__ ulonglong2 res_lane_take(ulonglong2 v, int addr) {
+  const auto take = [addr](uint64_t w) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_ds_bpermute(addr, (int)(uint32_t)w);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_ds_bpermute(addr, (int)(uint32_t)(w >> 32));
+    return ((uint64_t)hi << 32) | lo;
+  };
+  return make_ulonglong2(take(v.x), take(v.y));
+}
+
+template <int W>
+struct ResXState {
+  uint64_t s[2][W];         // own spins
+  ulonglong2 dm[2][W];      // own demons (after the walk)
+  ulonglong2 nx[W];         // the next colour's drawn demons
+};
+
+template <int W, int C>
+__device__ __forceinline__ void resx_colour(const ResArgs &a, ResXState<W> &st, uint64_t *ls,
+                                            const uint64_t *lj, uint32_t m, bool walk,
+                                            int src) {
+  const Geo &g = a.g;
+  const uint32_t H = (uint32_t)g.H, tid = threadIdx.x, bd = blockDim.x;
+  const bool drawn = a.slot[m] >= 0;
+  ulonglong2 d[W];
+#pragma unroll
+  for (int k = 0; k < W; ++k) d[k] = drawn ? st.nx[k] : st.dm[C][k];
+  {
+    const uint32_t m2 = C == 0 ? m : m + 1;
+    if (m2 < a.n && a.slot[m2] >= 0) {
+      const size_t pw = (size_t)g.R * g.V;
+      const ulonglong2 *p = a.pre + (size_t)a.slot[m2] * pw +
+                            ((size_t)blockIdx.x * 2 + (C ^ 1)) * H;
+#pragma unroll
+      for (int k = 0; k < W; ++k) {
+        const uint32_t h = tid + (uint32_t)k * bd;
+        if (h < H) st.nx[k] = p[h];
+      }
+    }
+  }
+  const uint64_t *So = ls + (size_t)(C ^ 1) * H;
+  const uint64_t *Jc = lj + (size_t)C * 3 * H, *Jo = lj + (size_t)(C ^ 1) * 3 * H;
+#pragma unroll
+  for (int k = 0; k < W; ++k) {
+    const uint32_t h = tid + (uint32_t)k * bd;
+    if (h >= H) continue;
+    const ResNb nb = res_nb(g, C, h, a.mLh, a.mL);
+    uint64_t D0 = d[k].x, D1 = d[k].y;
+    const uint64_t sn = msa::site_update(st.s[C][k], So[nb.xp], So[nb.yp], So[nb.zp], So[nb.xm],
+                                         So[nb.ym], So[nb.zm], Jc[h], Jc[H + h], Jc[2 * H + h],
+                                         Jo[nb.xm], Jo[H + nb.ym], Jo[2 * H + nb.zm], D0, D1);
+    st.s[C][k] = sn;
+    ls[(size_t)C * H + h] = sn;
+    d[k] = make_ulonglong2(D0, D1);
+  }
+  // the walk: colour 0 moves +x, colour 1 -x (wrapping in the row), every
+  // lane taking part; skipped where the next iteration's refresh replaces them
+  if (walk) {
+#pragma unroll
+    for (int k = 0; k < W; ++k) st.dm[C][k] = res_lane_take(d[k], src);
+  }
+  __syncthreads();
+}
+
+template <int W>
+__global__ void __launch_bounds__(1024) k_sweep_resident_x(ResArgs a) {
+  extern __shared__ uint4 s_res[];
+  const Geo &g = a.g;
+  const uint32_t H = (uint32_t)g.H, Lh = g.Lh, tid = threadIdx.x, bd = blockDim.x;
+  const uint32_t r = blockIdx.x;
+  uint64_t *ls = reinterpret_cast<uint64_t *>(s_res);   // spins [2][H]
+  uint64_t *lj = ls + 2 * (size_t)H;                     // J [2 colours][3][H]
+  const uint64_t *spi = a.sp_in + 2 * (size_t)r * H;
+  const ulonglong2 *dmi = a.dm_in + 2 * (size_t)r * H;
+  for (uint32_t q = tid; q < 2 * H; q += bd) ls[q] = spi[q];
+  for (uint32_t q = tid; q < 6 * H; q += bd) lj[q] = a.jp[q];
+  ResXState<W> st;
+#pragma unroll
+  for (int c = 0; c < 2; ++c)
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+      const uint32_t h = tid + (uint32_t)k * bd;
+      st.s[c][k] = h < H ? spi[(size_t)c * H + h] : 0;
+      st.dm[c][k] = (a.dm_load && h < H) ? dmi[(size_t)c * H + h] : make_ulonglong2(0, 0);
+    }
+  if (a.n > 0 && a.slot[0] >= 0) {
+    const ulonglong2 *p = a.pre + (size_t)a.slot[0] * ((size_t)g.R * g.V) + (size_t)r * 2 * H;
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+      const uint32_t h = tid + (uint32_t)k * bd;
+      if (h < H) st.nx[k] = p[h];
+    }
+  }
+  // lane of the word each word's walked demon comes from: x - 1 for colour
+  // 0, x + 1 for colour 1, in the same row of L/2 consecutive lanes
+  const uint32_t lane = __lane_id(), i = tid & (Lh - 1);
+  const int src0 = (int)(i ? lane - 1 : lane + Lh - 1) * 4;
+  const int src1 = (int)(i + 1 < Lh ? lane + 1 : lane + 1 - Lh) * 4;
+  __syncthreads();
+  for (uint32_t m = 0; m < a.n; ++m) {
+    const bool walk = m + 1 < a.n ? a.slot[m + 1] < 0 : a.dm_store != 0;
+    resx_colour<W, 0>(a, st, ls, lj, m, walk, src0);
+    resx_colour<W, 1>(a, st, ls, lj, m, walk, src1);
+  }
+  uint64_t *spo = a.sp_out + 2 * (size_t)r * H;
+  ulonglong2 *dmo = a.dm_out + 2 * (size_t)r * H;
+#pragma unroll
+  for (int c = 0; c < 2; ++c)
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+      const uint32_t h = tid + (uint32_t)k * bd;
+      if (h >= H) continue;
+      spo[(size_t)c * H + h] = st.s[c][k];
+      if (a.dm_store) dmo[(size_t)c * H + h] = st.dm[c][k];
+    }
 }
 
 struct MeasArgs {
@@ -1467,6 +1594,10 @@ extern "C" int msa_create(int device, const msa_geom *geom, msa_ctx **out) {
       hipFuncSetAttribute(reinterpret_cast<const void *>(&k_sweep_resident<1>),
                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)kResLds) != hipSuccess ||
       hipFuncSetAttribute(reinterpret_cast<const void *>(&k_sweep_resident<2>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)kResLds) != hipSuccess ||
+      hipFuncSetAttribute(reinterpret_cast<const void *>(&k_sweep_resident_x<1>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)kResLds) != hipSuccess ||
+      hipFuncSetAttribute(reinterpret_cast<const void *>(&k_sweep_resident_x<2>),
                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)kResLds) != hipSuccess)
     return cleanup(msa_fail(MSA_ERR_HIP, "kernel LDS attribute"));
   pick_half(c);
@@ -1913,7 +2044,12 @@ static void launch_resident(msa_ctx *c, ResArgs &ra) {
   ra.mLh = div_recip_of(g.Lh);
   ra.mL = div_recip_of(g.L);
   const size_t lds = res_lds(g);
-  if (H <= 1024)
+  if (res_x(g)) {
+    if (H <= 1024)
+      hipLaunchKernelGGL(k_sweep_resident_x<1>, dim3(g.R), dim3((H + 63) / 64 * 64), lds, c->stream, ra);
+    else
+      hipLaunchKernelGGL(k_sweep_resident_x<2>, dim3(g.R), dim3(1024), lds, c->stream, ra);
+  } else if (H <= 1024)
     hipLaunchKernelGGL(k_sweep_resident<1>, dim3(g.R), dim3((H + 63) / 64 * 64), lds, c->stream, ra);
   else
     hipLaunchKernelGGL(k_sweep_resident<2>, dim3(g.R), dim3(1024), lds, c->stream, ra);
