@@ -885,7 +885,7 @@ struct ResArgs {
 // Site h of colour col: its neighbours' indices in the other colour's plane
 // (the indexing of k_halfsweep) and its coordinates for the walk.
 struct ResNb {
-  uint32_t xp, yp, zp, xm, ym, zm, i, y, z;
+  uint32_t xp, yp, zp, xm, ym, zm, i, y, z, off;
 };
 __device__ __forceinline__ ResNb res_nb(const Geo &g, int col, uint32_t h, uint32_t mLh,
                                         uint32_t mL) {
@@ -907,6 +907,7 @@ __device__ __forceinline__ ResNb res_nb(const Geo &g, int col, uint32_t h, uint3
   n.i = i;
   n.y = y;
   n.z = z;
+  n.off = off;
   return n;
 }
 
@@ -1124,9 +1125,13 @@ __device__ __forceinline__ void resx_colour(const ResArgs &a, ResXState<W> &st, 
     if (h >= H) continue;
     const ResNb nb = res_nb(g, C, h, a.mLh, a.mL);
     uint64_t D0 = d[k].x, D1 = d[k].y;
-    const uint64_t sn = msa::site_update(st.s[C][k], So[nb.xp], So[nb.yp], So[nb.zp], So[nb.xm],
-                                         So[nb.ym], So[nb.zm], Jc[h], Jc[H + h], Jc[2 * H + h],
-                                         Jo[nb.xm], Jo[H + nb.ym], Jo[2 * H + nb.zm], D0, D1);
+    // one x neighbour is the other colour's word at the same index, which
+    // this thread holds (off = 0: the +x one, off = 1: the -x one)
+    const uint64_t near = st.s[C ^ 1][k], far = So[nb.off ? nb.xp : nb.xm];
+    const uint64_t sn = msa::site_update(st.s[C][k], nb.off ? far : near, So[nb.yp], So[nb.zp],
+                                         nb.off ? near : far, So[nb.ym], So[nb.zm], Jc[h],
+                                         Jc[H + h], Jc[2 * H + h], Jo[nb.xm], Jo[H + nb.ym],
+                                         Jo[2 * H + nb.zm], D0, D1);
     st.s[C][k] = sn;
     ls[(size_t)C * H + h] = sn;
     d[k] = make_ulonglong2(D0, D1);
