@@ -1,4 +1,5 @@
-"""Resident launches of small lattices (tiling mode 4, k_sweep_resident): up
+"""Resident launches of small lattices (tiling mode 4, k_sweep_resident and,
+for one layer with x rows that tile a wavefront, k_sweep_resident_x): up
 to 64 iterations per launch with the state in one CU's LDS, refreshes read
 from the demons drawn ahead (refresh modes 1 and 2) or drawn inside the
 sweep by the two-pass kernel between launches (mode 0).  Bit-exact against
@@ -22,6 +23,7 @@ pytestmark = pytest.mark.gpu
     (12, 2, 3, -1, 100, (1, 99)),       # t = 0, 1, 2, 4, ...
     (10, 1, 6, 2, 20, (9, 11)),         # six layers, as the reference's workload
     (8, 3, 2, 0, 130, (130,)),          # no refresh: three launches
+    (8, 3, 1, 1, 70, (33, 37)),         # demons in registers, one word per thread
     (6, 2, 2, 1, 30, (13, 17)),
     (4, 1, 1, 1, 20, (20,)),            # 32 sites of a colour in a 64-thread block
 ])
