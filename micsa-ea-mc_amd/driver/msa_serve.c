@@ -5,6 +5,7 @@
  *
  * usage: msa_serve requests.txt results.txt [--max-bytes N] [--epoch N]
  *                  [--block B] [--workers W] [--state-dir DIR] [--solo-min-L N]
+ *                  [--solo-min-iter N] [--solo-share X]
  *
  * requests.txt   one request per line: rid L R niter T0 T1 seed arrival_s
  *                (arrival in seconds after the start of the stream; '#' starts
@@ -32,6 +33,14 @@
  *                0 = every request in the batch (default 128: at L = 128
  *                the plain form, its refresh drawn ahead, runs 1.4 times the
  *                batch kernel's rate; at L = 96 the two are within 5 %)
+ * --solo-min-iter  only requests of at least N sweeps go solo (default 1024:
+ *                a solo request pays its own start -- tables, a new draw
+ *                pipeline -- which short requests do not amortise)
+ * --solo-share   an eligible request goes solo while the solo lane's pending
+ *                work (sites x sweeps of its unfinished requests) is at most X
+ *                times the batch's, else to the batch, so that one lane does
+ *                not finish long after the other (default 2; 0 = every
+ *                eligible request solo)
  *
  * A request is admitted, in arrival order, once it has arrived, its tables
  * are built and its footprint fits next to the lattices on the device; every
@@ -280,6 +289,12 @@ static void write_result(FILE *out, request *q, const char *state_dir, msa_ctx *
   fprintf(out, "\n");
 }
 
+/* A request's work: sites x replicas x sweeps. */
+static double work_of(const request *q)
+{
+  return (double)q->R * q->L * q->L * q->L * (double)q->niter;
+}
+
 /* ------------------------------------------------------------------ main -- */
 static int cmp_double(const void *a, const void *b)
 {
@@ -312,13 +327,16 @@ int main(int argc, char **argv)
 {
   if (argc < 3) {
     fprintf(stderr, "usage: %s requests.txt results.txt [--max-bytes N] [--epoch N] [--block B] "
-                    "[--workers W] [--state-dir DIR] [--solo-min-L N]\n", argv[0]);
+                    "[--workers W] [--state-dir DIR] [--solo-min-L N] [--solo-min-iter N] "
+                    "[--solo-share X]\n", argv[0]);
     return 1;
   }
   int64_t max_bytes = (int64_t)8 << 30;
   int epoch = 64, block = 0, workers = 4;
   const char *state_dir = NULL;
   int solo_min_L = 128;
+  int64_t solo_min_iter = 1024;
+  double solo_share = 2.0;
   for (int a = 3; a < argc; a++) {
     const char *opt = argv[a];
     if (a + 1 >= argc) fail("option %s needs a value", opt);
@@ -329,9 +347,13 @@ int main(int argc, char **argv)
     else if (!strcmp(opt, "--workers")) workers = atoi(v);
     else if (!strcmp(opt, "--state-dir")) state_dir = v;
     else if (!strcmp(opt, "--solo-min-L")) solo_min_L = atoi(v);
+    else if (!strcmp(opt, "--solo-min-iter")) solo_min_iter = strtoll(v, NULL, 10);
+    else if (!strcmp(opt, "--solo-share")) solo_share = atof(v);
     else fail("unknown option %s", opt);
   }
-  if (epoch < 1 || workers < 1 || max_bytes < 1 || solo_min_L < 0) fail("bad option value");
+  if (epoch < 1 || workers < 1 || max_bytes < 1 || solo_min_L < 0 || solo_min_iter < 0 ||
+      solo_share < 0)
+    fail("bad option value");
 
   /* requests */
   FILE *f = fopen(argv[1], "r");
@@ -435,6 +457,7 @@ int main(int argc, char **argv)
   int nactive = 0, done = 0, nxt = 0, max_active = 0, max_slot = 0;
   int64_t max_bytes_used = 0;
   double updates = 0;
+  double solo_pending = 0, batch_pending = 0;   /* work of each lane's unfinished requests */
   const double t_start = now_s();
   double last_note = t_start;
 
@@ -442,8 +465,16 @@ int main(int argc, char **argv)
     const double now = now_s() - t_start;
     while (nxt < n && reqs[nxt].arrival <= now) {
       submit(&p, nxt);
-      if (solo_min_L > 0 && reqs[nxt].L >= solo_min_L) sq[sqt++] = nxt;
-      else queue[qt++] = nxt;
+      const request *r = &reqs[nxt];
+      const double w = work_of(r);
+      if (solo_min_L > 0 && r->L >= solo_min_L && r->niter >= solo_min_iter &&
+          (solo_share == 0 || solo_pending <= solo_share * batch_pending)) {
+        sq[sqt++] = nxt;
+        solo_pending += w;
+      } else {
+        queue[qt++] = nxt;
+        batch_pending += w;
+      }
       nxt++;
     }
     /* the solo lane: finish the running request, queue its next chunk, or
@@ -460,6 +491,7 @@ int main(int argc, char **argv)
         q->finish = now_s() - t_start;
         write_result(out, q, state_dir, solo, NULL, -1);
         free_tables(q);
+        solo_pending -= work_of(q);
         done++;
         updates += 64.0 * q->R * (double)q->L * q->L * q->L * (double)q->niter;
         solo_cur = -1;
@@ -556,6 +588,7 @@ int main(int argc, char **argv)
       q->finish = now_s() - t_start;
       write_result(out, q, state_dir, NULL, b, s);
       CHK(msa_batch_remove(b, s));
+      batch_pending -= work_of(q);
       slot_req[s] = -1;
       nactive--;
       done++;
@@ -586,9 +619,9 @@ int main(int argc, char **argv)
   printf("{\"requests\": %d, \"makespan_s\": %.6f, \"spin_updates_per_ps\": %.6f, "
          "\"latency_p50_s\": %.6f, \"latency_p99_s\": %.6f, \"max_concurrent\": %d, "
          "\"max_bytes_in_use\": %lld, \"epoch\": %d, \"block\": %d, \"solo_requests\": %d, "
-         "\"solo_min_L\": %d}\n",
+         "\"solo_min_L\": %d, \"solo_min_iter\": %lld, \"solo_share\": %g}\n",
          n, makespan, updates / makespan / 1e12, p50, p99, max_active, (long long)max_bytes_used,
-         epoch, block, sqt, solo_min_L);
+         epoch, block, sqt, solo_min_L, (long long)solo_min_iter, solo_share);
   for (int i = 0; i < n; i++) free(reqs[i].energy);
   free(reqs);
   free(luts);

@@ -134,7 +134,7 @@ def run_alone(req, tables, jobid):
 
 
 def serve(requests, max_bytes, epoch=64, block=None, tables=None, keep_state=False, workers=4,
-          progress=None, solo_min_L=128):
+          progress=None, solo_min_L=128, solo_min_iter=1024, solo_share=2.0):
     """Run the stream through the native request loop (bin/msa_serve, the
     C driver of the served form); returns (per-request dicts, summary dict).
     block=None lets the driver's autotuner pick the batch kernel's block size
@@ -144,8 +144,10 @@ def serve(requests, max_bytes, epoch=64, block=None, tables=None, keep_state=Fal
     returns each request's final spins and demons.  progress, if given, gets
     the driver's progress lines.  `tables` is accepted for callers that hold
     one; the driver builds the same temperature grid itself.  Requests with
-    L >= solo_min_L run one at a time as plain lattices beside the batch
-    (0: every request in the batch)."""
+    L >= solo_min_L and at least solo_min_iter sweeps run one at a time as
+    plain lattices beside the batch (0: every request in the batch), while
+    the solo lane's pending work is at most solo_share times the batch's
+    (0: always)."""
     import json
     import os
     import subprocess
@@ -162,7 +164,8 @@ def serve(requests, max_bytes, epoch=64, block=None, tables=None, keep_state=Fal
         res = os.path.join(d, "results.txt")
         cmd = [SERVE_BIN, rq, res, "--max-bytes", str(int(max_bytes)), "--epoch", str(epoch),
                "--block", str(block or 0), "--workers", str(workers),
-               "--solo-min-L", str(int(solo_min_L))]
+               "--solo-min-L", str(int(solo_min_L)), "--solo-min-iter", str(int(solo_min_iter)),
+               "--solo-share", repr(float(solo_share))]
         if keep_state:
             os.makedirs(os.path.join(d, "state"))
             cmd += ["--state-dir", os.path.join(d, "state")]

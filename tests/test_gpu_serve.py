@@ -83,14 +83,16 @@ def test_serving_mix_equals_alone():
     assert summary["latency_p99_s"] >= summary["latency_p50_s"] > 0
 
 
-def test_served_configuration_subset(orc):
+@pytest.mark.parametrize("share", [0.0, 2.0])
+def test_served_configuration_subset(orc, share):
     """The served configuration's mix on 100 requests at short lengths: L
     uniform over {16, 32, 48, 64, 96, 128}, 64 +-J samples, 2^5..2^8 sweeps
     from T = 2.0 to a final T uniform in [0.6, 0.9], Poisson arrivals (seed
     5151), admission by footprint under a limit of three L = 128 lattices, so
-    requests queue and overlap.  Every request's spins, demons and energies
-    equal the request run alone on the plain path; the L <= 32 requests also
-    equal the oracle."""
+    requests queue and overlap; every L = 128 request on the solo lane
+    (share 0), or routed by the balance of pending work (share 2).  Every
+    request's spins, demons and energies equal the request run alone on the
+    plain path; the L <= 32 requests also equal the oracle."""
     serve, msa = modules()
     reqs = serve.make_requests(100, rate=2000.0, seed=5151, log2_len=(5, 8))
     assert {r.L for r in reqs} == {16, 32, 48, 64, 96, 128}
@@ -99,15 +101,21 @@ def test_served_configuration_subset(orc):
     limit = 3 * max(probe.job_bytes(tables.job(r, r.rid)) for r in reqs)
     probe.close()
     done, summary = serve.serve(reqs, max_bytes=limit, epoch=32, block=1024, tables=tables,
-                                keep_state=True)
-    print("served %d requests: %.3f s, max %d resident, %.2f spin updates/ps" % (
+                                keep_state=True, solo_min_iter=0, solo_share=share)
+    print("served %d requests: %.3f s, max %d resident, %d solo, %.2f spin updates/ps" % (
         summary["requests"], summary["makespan_s"], summary["max_concurrent"],
-        summary["spin_updates_per_ps"]), flush=True)
+        summary["solo_requests"], summary["spin_updates_per_ps"]), flush=True)
     assert summary["requests"] == len(reqs)
     assert summary["max_bytes_in_use"] <= limit
     assert summary["max_concurrent"] >= 3
-    # L = 128 ran one at a time as plain lattices beside the batch
-    assert summary["solo_requests"] == sum(r.L >= 128 for r in reqs) > 0
+    n128 = sum(r.L >= 128 for r in reqs)
+    if share == 0:
+        # every L = 128 request ran one at a time as a plain lattice beside the batch
+        assert summary["solo_requests"] == n128 > 0
+    else:
+        # some went solo, the others to the batch when the solo lane had more
+        # than its share of the pending work
+        assert 0 < summary["solo_requests"] <= n128
     check_against_alone(serve, tables, reqs, done)
     small = [r for r in reqs if r.L <= 32]
     assert len(small) >= 20
@@ -130,7 +138,7 @@ def test_solo_lane_in_chunks():
             for i, (L, n) in enumerate([(16, 4100), (24, 2048), (16, 2049), (24, 700), (16, 5)])]
     tables = serve.Tables()
     done, summary = serve.serve(reqs, max_bytes=1 << 30, epoch=16, block=256, tables=tables,
-                                keep_state=True, solo_min_L=16)
+                                keep_state=True, solo_min_L=16, solo_min_iter=0, solo_share=0)
     assert summary["requests"] == len(reqs) and summary["solo_requests"] == len(reqs)
     assert summary["max_concurrent"] == 0                 # nothing went to the batch
     check_against_alone(serve, tables, reqs, done)
