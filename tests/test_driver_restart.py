@@ -196,7 +196,7 @@ def assert_same_outputs(outA, outB, lanes, R, sample=3, restarted=True):
 # msa_destroy and process exit -- and for the other two cases; what remains
 # untested is the HIP runtime's own teardown.  No cause is shown, so the case
 # is not run until one is; the same annealed maxtime restart runs below on
-# L = 8 with one layer.
+# L = 8 with one layer and L = 24 with two.
 ANNEALED_EXIT = pytest.mark.skip(reason="driver process ended with SIGSEGV after its last line; "
                                         "cause not found")
 
@@ -205,11 +205,12 @@ ANNEALED_EXIT = pytest.mark.skip(reason="driver process ended with SIGSEGV after
 @pytest.mark.parametrize("nterm,sched,L,nc", [(0, None, 16, 2), (3, "0\n0\n256\n-1\n", 16, 2),
                                               pytest.param(0, "2.0\n0.7\n256\n4\n", 16, 2,
                                                            marks=ANNEALED_EXIT),
-                                              (0, "2.0\n0.7\n256\n4\n", 8, 1)])
+                                              (0, "2.0\n0.7\n256\n4\n", 8, 1),
+                                              (0, "2.0\n0.7\n256\n4\n", 24, 2)])
 def test_restart_after_maxtime_continues_bit_for_bit(tmp_path, nterm, sched, L, nc):
     """The reference's own workload (no schedule file, walker streams), and
     the Philox-refresh runs at fixed beta with thermalization and annealed
-    (L = 16 with two layers; L = 8 with one, on the resident kernel)."""
+    (L = 16 and 24 with two layers; L = 8 with one, on the resident kernel)."""
     R = 2
     times = tuple(2 ** k for k in range(9))            # 1 .. 256
     extra = []
