@@ -115,18 +115,16 @@ __device__ __forceinline__ void philox10(uint32_t &v0, uint32_t &v1, uint32_t &v
 // umbral[n-1] <= R < umbral[n].
 constexpr int kPreBits = 15;
 struct FineLut {
-  uint32_t umb[512];                       // 256 x (msb32, lsb32)
+  uint64_t umb[256];                       // umbral[n] = msb32 << 32 | lsb32
   uint8_t pre[(1u << kPreBits) + 16];      // + the sentinel 255, padded
 };
 static_assert(sizeof(FineLut) % 16 == 0, "FineLut is copied in 16-byte units");
 
 struct LutView {
-  const uint2 *umb;
+  const uint64_t *umb;
   const uint8_t *pre;
 };
-__device__ __forceinline__ LutView lut_view(const FineLut *t) {
-  return LutView{reinterpret_cast<const uint2 *>(t->umb), t->pre};
-}
+__device__ __forceinline__ LutView lut_view(const FineLut *t) { return LutView{t->umb, t->pre}; }
 
 // Byte n with umbral[n-1] <= R < umbral[n], R = (rm:rl): the bisection of
 // main.cu:615-630 started from the bracket [pre[b], pre[b+1]] of R's bucket
@@ -136,10 +134,8 @@ __device__ __forceinline__ LutView lut_view(const FineLut *t) {
 // reference bisection on every interval endpoint and bucket edge, and the
 // device's refresh words equal the oracle's in every refresh parity test.
 __device__ __forceinline__ uint32_t bisect(const LutView &t, uint32_t rm, uint32_t rl) {
-  auto below = [&](uint32_t n) {
-    const uint2 u = t.umb[n];
-    return (rm == u.x) ? (rl < u.y) : (rm < u.x);
-  };
+  const uint64_t r = ((uint64_t)rm << 32) | rl;
+  auto below = [&](uint32_t n) { return r < t.umb[n]; };
   const uint32_t b = rm >> (32 - kPreBits);
   uint32_t lo = t.pre[b], hi = t.pre[b + 1];
   if (lo == hi) return lo;                 // no threshold inside the bucket
@@ -186,10 +182,7 @@ __device__ __forceinline__ uint32_t rnd_word(const LutView &t, uint32_t c[4], ui
 __device__ __forceinline__ uint32_t bracket_search(const LutView &t, uint32_t rm, uint32_t rl,
                                                    uint32_t lo, uint32_t hi) {
   const uint64_t r = ((uint64_t)rm << 32) | rl;
-  auto below = [&](uint32_t n) {
-    const uint2 u = t.umb[n];
-    return r < (((uint64_t)u.x << 32) | u.y);
-  };
+  auto below = [&](uint32_t n) { return r < t.umb[n]; };
   for (int it = 0; it < 9 && hi - lo > 1; ++it) {
     const uint32_t mid = (hi + lo) >> 1;
     if (below(mid)) hi = mid; else lo = mid;

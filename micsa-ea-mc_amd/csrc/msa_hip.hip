@@ -253,7 +253,7 @@ __global__ void __launch_bounds__(256) k_fine_luts(const msa_lut *luts, uint32_t
     if (R < t) hi = m; else lo = m + 1;
   }
   out[tab].pre[b] = (uint8_t)lo;
-  if (b < 512) out[tab].umb[b] = u[b];
+  if (b < 256) out[tab].umb[b] = ((uint64_t)u[2 * b] << 32) | u[2 * b + 1];
 }
 
 // d_gen_rndbits_cuda (main.cu:647-702) with NBETAS = 1: thread tid < T =
