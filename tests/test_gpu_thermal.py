@@ -19,7 +19,8 @@ def random_state(L, R, seed):
 
 
 @pytest.mark.parametrize("L,R,nterm,beta", [(8, 1, 3, 0.5), (16, 2, 4, 0.9), (24, 1, 2, 0.3),
-                                            (32, 3, 2, 1.2)])
+                                            (32, 3, 2, 1.2),
+                                            (16, 2, 3, 2.0)])   # T = 0.5: wide search brackets
 def test_thermal_matches_oracle(pkg, orc, L, R, nterm, beta):
     print("case L=%d R=%d nterm=%d beta=%g" % (L, R, nterm, beta), flush=True)
     o = random_state(L, R, 100 + L)
